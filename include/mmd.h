@@ -1,0 +1,146 @@
+/*
+ * mmd.h -- C ABI of libmmd, the MI355X (gfx950) McMurchie-Davidson
+ * electron-repulsion-integral and Fock-build library.
+ *
+ * Plain pointers and sizes only.  Matrices are row-major float64; an N x N
+ * matrix has element (i,j) at [i*N+j]; the ERI tensor is chemists' (ij|kl)
+ * at [((i*N+j)*N+k)*N+l].  Basis functions are the reference's Cartesian
+ * components, in the reference's order (atoms -> shells -> components
+ * x, y, z / xx, xy, xz, yy, yz, zz).
+ *
+ * Every call returns 0 (MMD_OK) or a negative MMD_ERR_* code; the message of
+ * the last failure on the calling thread is returned by mmd_last_error().
+ * All calls are synchronous: outputs are complete when the call returns.
+ * A context belongs to one host thread at a time; calls on different
+ * contexts may run concurrently.
+ */
+#ifndef MMD_H
+#define MMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MMD_OK 0
+#define MMD_ERR_ARG (-1)         /* invalid argument (Python: ValueError)          */
+#define MMD_ERR_NOMEM (-3)       /* device or host allocation failed (MemoryError) */
+#define MMD_ERR_DEVICE (-4)      /* HIP runtime / no device (RuntimeError)         */
+#define MMD_ERR_STATE (-5)       /* call order: e.g. no basis set yet              */
+#define MMD_ERR_UNSUPPORTED (-6) /* angular momentum above d, incomplete shells    */
+
+typedef struct mmd_ctx mmd_ctx;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *mmd_last_error(void);
+
+/* Library ABI version (this header: 1). */
+int mmd_version(void);
+
+/* Number of visible HIP devices (0 without a GPU). */
+int mmd_device_count(int *count);
+
+/* Create / destroy a context on one device (one HIP stream per context). */
+int mmd_ctx_create(mmd_ctx **out, int device_id);
+void mmd_ctx_destroy(mmd_ctx *ctx);
+
+/*
+ * Basis functions, one per Cartesian component (replaces the list of
+ * `Basis` objects, cython/basis.pxi:6-73, built by Molecule.formBasis,
+ * mmd/molecule.py:57-73).
+ *   origin  : 3*nbf, bohr
+ *   lmn     : 3*nbf exponents (l, m, n), l+m+n <= 2
+ *   nprim   : nbf primitive counts
+ *   exps    : sum(nprim) exponents, basis function after basis function
+ *   weights : sum(nprim) products norm[k]*coefs[k] of the normalised
+ *             function (cython/basis.pxi:87-120)
+ * Replaces any previous basis and drops derived data (Schwarz, tensor).
+ */
+int mmd_set_basis(mmd_ctx *ctx, int nbf, const double *origin, const int32_t *lmn, const int32_t *nprim,
+                  const double *exps, const double *weights);
+
+/* Nuclei for the nuclear-attraction integrals: natom charges Z and 3*natom
+   coordinates (bohr). */
+int mmd_set_atoms(mmd_ctx *ctx, int natom, const double *Z, const double *xyz);
+
+/*
+ * One-electron integrals (Molecule.one_electron_integrals,
+ * mmd/molecule.py:235-281, with S/T/V/Mu/RxDel of cython/onee.pyx:14-69):
+ * S, T, V (sum over nuclei of -Z V_C): N*N each; M (dipole, origin `center`)
+ * and L (angular momentum, antisymmetric): 3*N*N each.  Host pointers.
+ */
+int mmd_one_electron(mmd_ctx *ctx, const double center[3], double *S, double *T, double *V, double *M,
+                     double *L);
+
+/*
+ * Schwarz diagonal (pq|pq) for every pair (the `screen` dictionary of
+ * Molecule.build(direct=True), mmd/molecule.py:93-99), as a full symmetric
+ * N*N host matrix (may be NULL).  Also builds the shell-pair tables and
+ * quartet bounds used by mmd_fock_delta.
+ */
+int mmd_schwarz(mmd_ctx *ctx, double *diag);
+
+/*
+ * Install a caller-supplied Schwarz diagonal (N*N, (pq|pq)) instead of the
+ * computed one -- the `screen` argument of formPT (cython/fock.pyx:13).
+ */
+int mmd_set_schwarz(mmd_ctx *ctx, const double *diag);
+
+/*
+ * Full ERI tensor with 8-fold symmetry (doERIs, cython/twoe.pyx:9-31).
+ * out: N^4 doubles, host memory (out_on_device = 0) or device memory
+ * (out_on_device = 1).
+ */
+int mmd_eri_tensor(mmd_ctx *ctx, double *out, int out_on_device);
+
+/*
+ * Contracted ERIs of arbitrary basis-function quartets of the current basis
+ * (ERI(a,b,c,d), cython/twoe.pyx:34-50): ijkl holds 4*n indices, out n
+ * values.  Host pointers.
+ */
+int mmd_eri_list(mmd_ctx *ctx, int64_t n, const int64_t *ijkl, double *out);
+
+/*
+ * Direct incremental two-electron Fock contribution G(dP), dP = P - P_old
+ * (formPT, cython/fock.pyx:10-87).  Needs mmd_schwarz first.
+ *   dP_re, dP_im : N*N real / imaginary parts (dP_im may be NULL)
+ *   tol          : screening threshold (acc2e)
+ *   raw = 1      : formPT's own unsymmetrised output (per canonical
+ *                  basis-function quartet, any dP)
+ *   raw = 0      : the symmetrised matrix 0.5*(G + G^T), as mmd/scf.py:93
+ *                  forms it; requires real symmetric dP (dP_im == NULL)
+ *   G_re, G_im   : N*N outputs (G_im ignored if dP_im == NULL)
+ *   on_device    : all of dP_*, G_* are device pointers (1) or host (0)
+ *   nsurv        : if not NULL, receives the number of shell quartets that
+ *                  passed the shell-level screen
+ */
+int mmd_fock_delta(mmd_ctx *ctx, const double *dP_re, const double *dP_im, double tol, int raw, double *G_re,
+                   double *G_im, int on_device, int64_t *nsurv);
+
+/*
+ * Conventional J and K from a stored tensor (mmd/scf.py:97-98):
+ *   J_pq = sum_rs (pq|rs) P_sr,   K_pq = sum_rs (ps|qr) P_sr
+ * T: N^4 (host or device per t_on_device); P, J, K: N*N host pointers;
+ * P_im / J_im / K_im may be NULL for a real density.
+ */
+int mmd_jk_tensor(mmd_ctx *ctx, const double *T, int t_on_device, const double *P_re, const double *P_im,
+                  double *J_re, double *J_im, double *K_re, double *K_im);
+
+/*
+ * Boys function F_m(T) (boys, cython/util.pxi:53-55; test hook _boys,
+ * cython/onee.pyx:195-196), evaluated on the device for n arguments:
+ *   table = 0 : general real order m >= 0 (series / continued fraction)
+ *   table = 1 : the tabulated kernel path (integer m <= 14)
+ * Host pointers.
+ */
+int mmd_boys(mmd_ctx *ctx, int n, const double *m, const double *T, int table, double *out);
+
+/* Device time (ms) of the last mmd_fock_delta call, from HIP events. */
+int mmd_last_fock_ms(mmd_ctx *ctx, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MMD_H */

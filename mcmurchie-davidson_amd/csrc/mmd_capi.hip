@@ -1,0 +1,742 @@
+// libmmd host runtime: the C ABI declared in include/mmd.h.
+//
+// A context owns one HIP stream, the Boys table, the basis (per function and
+// per shell), shell-pair / primitive-pair tables, Schwarz data, the density
+// and Fock scratch buffers and the per-class survivor lists.  Callers own
+// every input and output pointer.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/mmd.h"
+#include "mmd_device.h"
+#include "mmd_kernels.h"
+#include "mmd_types.h"
+
+using namespace mmd;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                           \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess) {                                                                 \
+            if (e_ == hipErrorOutOfMemory) return fail(MMD_ERR_NOMEM, "%s: %s", #expr, hipGetErrorString(e_)); \
+            return fail(MMD_ERR_DEVICE, "%s: %s", #expr, hipGetErrorString(e_));                \
+        }                                                                                       \
+    } while (0)
+
+struct Shell {
+    int l, np, p0, bf0;
+    double x, y, z;
+};
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t reserve(size_t count)
+    {
+        if (count <= n && p) return hipSuccess;
+        release();
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+        if (e == hipSuccess) n = count;
+        return e;
+    }
+};
+
+}  // namespace
+
+struct mmd_ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_fock_ms = 0.0;
+
+    // basis, per function
+    int nbf = 0;
+    std::vector<BFDesc> bf;
+    std::vector<double> bf_exp, bf_w;
+    DevBuf<BFDesc> d_bf;
+    DevBuf<double> d_bf_exp, d_bf_w;
+    DevBuf<double> d_boys;
+
+    // basis, per shell
+    bool shells_ok = false;
+    std::string shells_err;
+    std::vector<Shell> shells;
+    std::vector<double> sh_exp, sh_w, gcomp;
+    DevBuf<double> d_gcomp;
+    DevBuf<int> d_sh_bf0, d_sh_nbf;
+
+    // shell pairs (grouped by pair class, sorted inside each class)
+    bool pairs_ok = false;
+    std::vector<ShellPair> pairs;
+    int pc_off[NPAIRCLASS + 1] = {0};
+    DevBuf<ShellPair> d_pairs;
+    DevBuf<PrimPair> d_pp;
+
+    // Schwarz
+    bool schwarz_ok = false;
+    DevBuf<double> d_schwarz, d_sqS;
+
+    // quartet classes
+    int64_t qc_total[NQCLASS] = {0};
+    int64_t qc_off[NQCLASS + 1] = {0};
+    DevBuf<int2> d_surv;
+    DevBuf<int> d_count;
+
+    // atoms
+    int natom = 0;
+    DevBuf<double> d_atoms;
+
+    // density / Fock scratch
+    DevBuf<double> d_D, d_Dim, d_absD, d_Dsh, d_G, d_Gout;
+};
+
+namespace {
+
+int check_ctx(mmd_ctx* ctx)
+{
+    if (!ctx) return fail(MMD_ERR_ARG, "null context");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return fail(MMD_ERR_DEVICE, "hipSetDevice: %s", hipGetErrorString(e));
+    return MMD_OK;
+}
+
+// Group consecutive basis functions into complete Cartesian shells in the
+// reference's component order with a common centre and exponents and
+// weights that differ by a per-component factor only.
+void build_shells(mmd_ctx* c)
+{
+    c->shells.clear();
+    c->sh_exp.clear();
+    c->sh_w.clear();
+    c->gcomp.assign(c->nbf, 1.0);
+    c->shells_ok = false;
+    int i = 0;
+    while (i < c->nbf) {
+        const BFDesc& b0 = c->bf[i];
+        const int L = b0.l + b0.m + b0.n;
+        const int nc = ncart(L);
+        if (b0.l != L || i + nc > c->nbf) {
+            c->shells_err = "basis functions do not form complete Cartesian shells";
+            return;
+        }
+        for (int k = 0; k < nc; ++k) {
+            const BFDesc& bk = c->bf[i + k];
+            bool ok = bk.l == comp_lx(L, k) && bk.m == comp_ly(L, k) && bk.n == comp_lz(L, k) && bk.np == b0.np
+                   && bk.x == b0.x && bk.y == b0.y && bk.z == b0.z;
+            if (ok) {
+                const double g = c->bf_w[bk.p0] / c->bf_w[b0.p0];
+                for (int p = 0; p < b0.np && ok; ++p) {
+                    ok = c->bf_exp[bk.p0 + p] == c->bf_exp[b0.p0 + p];
+                    const double gp = c->bf_w[bk.p0 + p] / c->bf_w[b0.p0 + p];
+                    ok = ok && fabs(gp - g) <= 1e-12 * fabs(g);
+                }
+                c->gcomp[i + k] = g;
+            }
+            if (!ok) {
+                c->shells_err = "basis functions do not form complete Cartesian shells";
+                return;
+            }
+        }
+        Shell s;
+        s.l = L;
+        s.np = b0.np;
+        s.p0 = (int)c->sh_exp.size();
+        s.bf0 = i;
+        s.x = b0.x;
+        s.y = b0.y;
+        s.z = b0.z;
+        for (int p = 0; p < b0.np; ++p) {
+            c->sh_exp.push_back(c->bf_exp[b0.p0 + p]);
+            c->sh_w.push_back(c->bf_w[b0.p0 + p]);
+        }
+        c->shells.push_back(s);
+        i += nc;
+    }
+    c->shells_ok = true;
+}
+
+int upload_shells(mmd_ctx* c)
+{
+    HIP_TRY(c->d_gcomp.reserve(c->nbf));
+    HIP_TRY(hipMemcpyAsync(c->d_gcomp.p, c->gcomp.data(), sizeof(double) * c->nbf, hipMemcpyHostToDevice, c->st));
+    const int ns = (int)c->shells.size();
+    std::vector<int> b0(ns), nb(ns);
+    for (int s = 0; s < ns; ++s) {
+        b0[s] = c->shells[s].bf0;
+        nb[s] = ncart(c->shells[s].l);
+    }
+    HIP_TRY(c->d_sh_bf0.reserve(ns));
+    HIP_TRY(c->d_sh_nbf.reserve(ns));
+    HIP_TRY(hipMemcpyAsync(c->d_sh_bf0.p, b0.data(), sizeof(int) * ns, hipMemcpyHostToDevice, c->st));
+    HIP_TRY(hipMemcpyAsync(c->d_sh_nbf.p, nb.data(), sizeof(int) * ns, hipMemcpyHostToDevice, c->st));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    return MMD_OK;
+}
+
+// Shell pairs I >= J in canonical orientation, with primitive-pair data.
+// `order` (optional): sort inside each pair class by (npp desc, q desc).
+int build_pairs(mmd_ctx* c, const std::vector<double>* qpair)
+{
+    const int ns = (int)c->shells.size();
+    std::vector<std::vector<ShellPair>> byclass(NPAIRCLASS);
+    std::vector<PrimPair> pp;
+    int pidx = 0;
+    for (int I = 0; I < ns; ++I)
+        for (int J = 0; J <= I; ++J, ++pidx) {
+            int a = I, b = J;
+            if (c->shells[I].l < c->shells[J].l) std::swap(a, b);
+            const Shell& A = c->shells[a];
+            const Shell& B = c->shells[b];
+            ShellPair sp;
+            sp.sa = a;
+            sp.sb = b;
+            sp.bfa = A.bf0;
+            sp.bfb = B.bf0;
+            sp.pp0 = (int)pp.size();
+            sp.npp = A.np * B.np;
+            sp.same = (a == b) ? 1 : 0;
+            sp.pad = pidx;
+            sp.q = qpair ? (*qpair)[pidx] : 1.0;
+            const double AB2 = (A.x - B.x) * (A.x - B.x) + (A.y - B.y) * (A.y - B.y) + (A.z - B.z) * (A.z - B.z);
+            for (int i = 0; i < A.np; ++i)
+                for (int j = 0; j < B.np; ++j) {
+                    const double ea = c->sh_exp[A.p0 + i], eb = c->sh_exp[B.p0 + j];
+                    PrimPair q;
+                    q.p = ea + eb;
+                    q.K = c->sh_w[A.p0 + i] * c->sh_w[B.p0 + j] * exp(-ea * eb / q.p * AB2);
+                    q.Px = (ea * A.x + eb * B.x) / q.p;
+                    q.Py = (ea * A.y + eb * B.y) / q.p;
+                    q.Pz = (ea * A.z + eb * B.z) / q.p;
+                    q.PAx = q.Px - A.x;
+                    q.PAy = q.Py - A.y;
+                    q.PAz = q.Pz - A.z;
+                    q.PBx = q.Px - B.x;
+                    q.PBy = q.Py - B.y;
+                    q.PBz = q.Pz - B.z;
+                    q.inv2p = 0.5 / q.p;
+                    q.pad = 0.0;
+                    pp.push_back(q);
+                }
+            byclass[pair_class(A.l, B.l)].push_back(sp);
+        }
+    c->pairs.clear();
+    for (int pc = 0; pc < NPAIRCLASS; ++pc) {
+        auto& v = byclass[pc];
+        if (qpair)
+            std::stable_sort(v.begin(), v.end(), [](const ShellPair& x, const ShellPair& y) {
+                if (x.npp != y.npp) return x.npp > y.npp;
+                return x.q > y.q;
+            });
+        c->pc_off[pc] = (int)c->pairs.size();
+        c->pairs.insert(c->pairs.end(), v.begin(), v.end());
+    }
+    c->pc_off[NPAIRCLASS] = (int)c->pairs.size();
+    HIP_TRY(c->d_pairs.reserve(c->pairs.size()));
+    HIP_TRY(c->d_pp.reserve(pp.size()));
+    HIP_TRY(hipMemcpyAsync(c->d_pairs.p, c->pairs.data(), sizeof(ShellPair) * c->pairs.size(), hipMemcpyHostToDevice,
+                           c->st));
+    HIP_TRY(hipMemcpyAsync(c->d_pp.p, pp.data(), sizeof(PrimPair) * pp.size(), hipMemcpyHostToDevice, c->st));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    // quartet-class sizes (implicit enumeration)
+    c->qc_off[0] = 0;
+    for (int qc = 0; qc < NQCLASS; ++qc) {
+        int pcb = 0;
+        while ((pcb + 1) * (pcb + 2) / 2 <= qc) ++pcb;
+        const int pck = qc - pcb * (pcb + 1) / 2;
+        const int64_t nb = c->pc_off[pcb + 1] - c->pc_off[pcb];
+        const int64_t nk = c->pc_off[pck + 1] - c->pc_off[pck];
+        c->qc_total[qc] = (pcb == pck) ? nb * (nb + 1) / 2 : nb * nk;
+        c->qc_off[qc + 1] = c->qc_off[qc] + c->qc_total[qc];
+    }
+    HIP_TRY(c->d_surv.reserve((size_t)std::max<int64_t>(c->qc_off[NQCLASS], (int64_t)c->pairs.size())));
+    HIP_TRY(c->d_count.reserve(NQCLASS));
+    c->pairs_ok = true;
+    return MMD_OK;
+}
+
+void class_of(int qc, int* pcb, int* pck)
+{
+    int b = 0;
+    while ((b + 1) * (b + 2) / 2 <= qc) ++b;
+    *pcb = b;
+    *pck = qc - b * (b + 1) / 2;
+}
+
+int grid_for(int64_t items)
+{
+    int64_t g = (items + 255) / 256;
+    if (g < 1) g = 1;
+    if (g > 2048) g = 2048;
+    return (int)g;
+}
+
+ClassArgs base_args(mmd_ctx* c)
+{
+    ClassArgs A;
+    memset(&A, 0, sizeof(A));
+    A.pairs = c->d_pairs.p;
+    A.pp = c->d_pp.p;
+    A.boys = c->d_boys.p;
+    A.gcomp = c->d_gcomp.p;
+    A.N = c->nbf;
+    return A;
+}
+
+// Screen (or take all) quartets of every class into the survivor lists.
+int screen_all(mmd_ctx* c, double tol, int keep_all)
+{
+    HIP_TRY(launch_fill_int(NQCLASS, c->d_count.p, 0, c->st));
+    for (int qc = 0; qc < NQCLASS; ++qc) {
+        if (c->qc_total[qc] == 0) continue;
+        int pcb, pck;
+        class_of(qc, &pcb, &pck);
+        const int nb = c->pc_off[pcb + 1] - c->pc_off[pcb];
+        const int nk = c->pc_off[pck + 1] - c->pc_off[pck];
+        HIP_TRY(launch_screen_class(c->pc_off[pcb], nb, c->pc_off[pck], nk, pcb == pck ? 1 : 0, c->d_pairs.p,
+                                    c->d_Dsh.p, (int)c->shells.size(), tol, keep_all, c->d_surv.p + c->qc_off[qc],
+                                    c->d_count.p + qc, c->st));
+    }
+    return MMD_OK;
+}
+
+int run_fock_classes(mmd_ctx* c, ClassArgs A)
+{
+    for (int qc = 0; qc < NQCLASS; ++qc) {
+        if (c->qc_total[qc] == 0) continue;
+        A.quart = c->d_surv.p + c->qc_off[qc];
+        A.count = c->d_count.p + qc;
+        const int64_t items = ((c->qc_total[qc] + 63) / 64) * 64 * fock_class_nchunk(qc);
+        HIP_TRY(launch_fock_class(qc, A, grid_for(items), c->st));
+    }
+    return MMD_OK;
+}
+
+}  // namespace
+
+namespace {
+// Install a Schwarz diagonal (N*N, (pq|pq)) and rebuild the sorted shell-pair
+// tables with their factors q = max sqrt((ab|ab)).
+int set_schwarz(mmd_ctx* ctx, const std::vector<double>& h)
+{
+    const int N = ctx->nbf;
+    const size_t nn = (size_t)N * N;
+    HIP_TRY(hipMemcpyAsync(ctx->d_schwarz.p, h.data(), sizeof(double) * nn, hipMemcpyHostToDevice, ctx->st));
+    HIP_TRY(launch_sqrt_clamped(nn, ctx->d_schwarz.p, ctx->d_sqS.p, ctx->st));
+    HIP_TRY(hipStreamSynchronize(ctx->st));
+    ctx->schwarz_ok = true;
+    if (!ctx->shells_ok) return MMD_OK;   // only the per-function paths are available
+    // pair factors q = max sqrt((ab|ab)) over the pair's functions, indexed by
+    // the pair's position in the I >= J enumeration
+    const int ns = (int)ctx->shells.size();
+    std::vector<double> q((size_t)ns * (ns + 1) / 2, 0.0);
+    int pidx = 0;
+    for (int I = 0; I < ns; ++I)
+        for (int J = 0; J <= I; ++J, ++pidx) {
+            const Shell& A1 = ctx->shells[I];
+            const Shell& B1 = ctx->shells[J];
+            double m = 0.0;
+            for (int a = 0; a < ncart(A1.l); ++a)
+                for (int b = 0; b < ncart(B1.l); ++b) {
+                    const double x = h[(size_t)(A1.bf0 + a) * N + B1.bf0 + b];
+                    const double s = x > 0.0 ? sqrt(x) : 0.0;
+                    if (s > m) m = s;
+                }
+            q[pidx] = m;
+        }
+    return build_pairs(ctx, &q);
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* mmd_last_error(void) { return g_last_error.c_str(); }
+
+int mmd_version(void) { return 1; }
+
+int mmd_device_count(int* count)
+{
+    if (!count) return fail(MMD_ERR_ARG, "null pointer");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *count = n;
+    return MMD_OK;
+}
+
+int mmd_ctx_create(mmd_ctx** out, int device_id)
+{
+    if (!out) return fail(MMD_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(MMD_ERR_DEVICE, "no HIP device available");
+    if (device_id < 0 || device_id >= n) return fail(MMD_ERR_ARG, "device %d out of range (%d devices)", device_id, n);
+    HIP_TRY(hipSetDevice(device_id));
+    mmd_ctx* c = new (std::nothrow) mmd_ctx();
+    if (!c) return fail(MMD_ERR_NOMEM, "context allocation failed");
+    c->device = device_id;
+    hipError_t e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    std::vector<double> tab;
+    boys_build_table(tab);
+    if (e == hipSuccess) e = c->d_boys.reserve(tab.size());
+    if (e == hipSuccess) e = hipMemcpy(c->d_boys.p, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        mmd_ctx_destroy(c);
+        return fail(MMD_ERR_DEVICE, "context setup: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return MMD_OK;
+}
+
+void mmd_ctx_destroy(mmd_ctx* ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->st) (void)hipStreamSynchronize(ctx->st);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    hipStream_t st = ctx->st;
+    delete ctx;  // frees device buffers
+    if (st) (void)hipStreamDestroy(st);
+}
+
+int mmd_set_basis(mmd_ctx* ctx, int nbf, const double* origin, const int32_t* lmn, const int32_t* nprim,
+                  const double* exps, const double* weights)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (nbf <= 0 || !origin || !lmn || !nprim || !exps || !weights) return fail(MMD_ERR_ARG, "invalid basis arguments");
+    ctx->bf.assign(nbf, BFDesc{});
+    ctx->bf_exp.clear();
+    ctx->bf_w.clear();
+    for (int i = 0; i < nbf; ++i) {
+        BFDesc& b = ctx->bf[i];
+        b.x = origin[3 * i];
+        b.y = origin[3 * i + 1];
+        b.z = origin[3 * i + 2];
+        b.l = lmn[3 * i];
+        b.m = lmn[3 * i + 1];
+        b.n = lmn[3 * i + 2];
+        if (b.l < 0 || b.m < 0 || b.n < 0) return fail(MMD_ERR_ARG, "negative angular exponent");
+        if (b.l + b.m + b.n > MAX_L)
+            return fail(MMD_ERR_UNSUPPORTED, "basis function %d has l=%d; this build supports l <= %d", i,
+                        b.l + b.m + b.n, MAX_L);
+        if (nprim[i] <= 0) return fail(MMD_ERR_ARG, "basis function %d has no primitives", i);
+        b.np = nprim[i];
+        b.p0 = (int)ctx->bf_exp.size();
+        b.pad = 0;
+        for (int p = 0; p < b.np; ++p) {
+            const double e = exps[b.p0 + p];
+            if (!(e > 0.0)) return fail(MMD_ERR_ARG, "non-positive exponent");
+            ctx->bf_exp.push_back(e);
+            ctx->bf_w.push_back(weights[b.p0 + p]);
+        }
+    }
+    ctx->nbf = nbf;
+    ctx->pairs_ok = false;
+    ctx->schwarz_ok = false;
+    HIP_TRY(ctx->d_bf.reserve(nbf));
+    HIP_TRY(ctx->d_bf_exp.reserve(ctx->bf_exp.size()));
+    HIP_TRY(ctx->d_bf_w.reserve(ctx->bf_w.size()));
+    HIP_TRY(hipMemcpyAsync(ctx->d_bf.p, ctx->bf.data(), sizeof(BFDesc) * nbf, hipMemcpyHostToDevice, ctx->st));
+    HIP_TRY(hipMemcpyAsync(ctx->d_bf_exp.p, ctx->bf_exp.data(), sizeof(double) * ctx->bf_exp.size(),
+                           hipMemcpyHostToDevice, ctx->st));
+    HIP_TRY(hipMemcpyAsync(ctx->d_bf_w.p, ctx->bf_w.data(), sizeof(double) * ctx->bf_w.size(), hipMemcpyHostToDevice,
+                           ctx->st));
+    HIP_TRY(hipStreamSynchronize(ctx->st));
+    build_shells(ctx);
+    if (ctx->shells_ok) {
+        rc = upload_shells(ctx);
+        if (rc) return rc;
+    }
+    const size_t nn = (size_t)nbf * nbf;
+    HIP_TRY(ctx->d_D.reserve(nn));
+    HIP_TRY(ctx->d_Dim.reserve(nn));
+    HIP_TRY(ctx->d_absD.reserve(nn));
+    HIP_TRY(ctx->d_G.reserve(nn));
+    HIP_TRY(ctx->d_Gout.reserve(nn));
+    HIP_TRY(ctx->d_schwarz.reserve(nn));
+    HIP_TRY(ctx->d_sqS.reserve(nn));
+    const size_t ns = ctx->shells.size();
+    HIP_TRY(ctx->d_Dsh.reserve(ns * ns));
+    return MMD_OK;
+}
+
+int mmd_set_atoms(mmd_ctx* ctx, int natom, const double* Z, const double* xyz)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (natom < 0 || (natom > 0 && (!Z || !xyz))) return fail(MMD_ERR_ARG, "invalid atoms");
+    std::vector<double> a(4 * (size_t)std::max(natom, 1));
+    for (int i = 0; i < natom; ++i) {
+        a[4 * i] = Z[i];
+        a[4 * i + 1] = xyz[3 * i];
+        a[4 * i + 2] = xyz[3 * i + 1];
+        a[4 * i + 3] = xyz[3 * i + 2];
+    }
+    HIP_TRY(ctx->d_atoms.reserve(a.size()));
+    HIP_TRY(hipMemcpy(ctx->d_atoms.p, a.data(), sizeof(double) * a.size(), hipMemcpyHostToDevice));
+    ctx->natom = natom;
+    return MMD_OK;
+}
+
+int mmd_one_electron(mmd_ctx* ctx, const double center[3], double* S, double* T, double* V, double* M, double* L)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (ctx->nbf == 0) return fail(MMD_ERR_STATE, "no basis set");
+    if (!center || !S || !T || !V || !M || !L) return fail(MMD_ERR_ARG, "null pointer");
+    const size_t nn = (size_t)ctx->nbf * ctx->nbf;
+    DevBuf<double> d;
+    HIP_TRY(d.reserve(9 * nn));
+    HIP_TRY(launch_one_electron(ctx->nbf, ctx->d_bf.p, ctx->d_bf_exp.p, ctx->d_bf_w.p, ctx->natom,
+                                ctx->natom ? ctx->d_atoms.p : nullptr, center[0], center[1], center[2], ctx->d_boys.p,
+                                d.p, d.p + nn, d.p + 2 * nn, d.p + 3 * nn, d.p + 6 * nn, ctx->st));
+    HIP_TRY(hipMemcpyAsync(S, d.p, sizeof(double) * nn, hipMemcpyDeviceToHost, ctx->st));
+    HIP_TRY(hipMemcpyAsync(T, d.p + nn, sizeof(double) * nn, hipMemcpyDeviceToHost, ctx->st));
+    HIP_TRY(hipMemcpyAsync(V, d.p + 2 * nn, sizeof(double) * nn, hipMemcpyDeviceToHost, ctx->st));
+    HIP_TRY(hipMemcpyAsync(M, d.p + 3 * nn, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, ctx->st));
+    HIP_TRY(hipMemcpyAsync(L, d.p + 6 * nn, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, ctx->st));
+    HIP_TRY(hipStreamSynchronize(ctx->st));
+    return MMD_OK;
+}
+
+int mmd_schwarz(mmd_ctx* ctx, double* diag)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (ctx->nbf == 0) return fail(MMD_ERR_STATE, "no basis set");
+    const int N = ctx->nbf;
+    const size_t nn = (size_t)N * N;
+    const int64_t npair = (int64_t)N * (N + 1) / 2;
+    std::vector<int64_t> idx(4 * npair);
+    int64_t k = 0;
+    for (int p = 0; p < N; ++p)
+        for (int q = 0; q <= p; ++q, ++k) {
+            idx[4 * k] = p;
+            idx[4 * k + 1] = q;
+            idx[4 * k + 2] = p;
+            idx[4 * k + 3] = q;
+        }
+    DevBuf<int64_t> d_idx;
+    DevBuf<double> d_val;
+    HIP_TRY(d_idx.reserve(4 * npair));
+    HIP_TRY(d_val.reserve(npair));
+    HIP_TRY(hipMemcpyAsync(d_idx.p, idx.data(), sizeof(int64_t) * 4 * npair, hipMemcpyHostToDevice, ctx->st));
+    HIP_TRY(launch_eri_bf_list(npair, d_idx.p, ctx->d_bf.p, ctx->d_bf_exp.p, ctx->d_bf_w.p, ctx->d_boys.p, d_val.p,
+                               ctx->st));
+    std::vector<double> v(npair), h(nn);
+    HIP_TRY(hipMemcpyAsync(v.data(), d_val.p, sizeof(double) * npair, hipMemcpyDeviceToHost, ctx->st));
+    HIP_TRY(hipStreamSynchronize(ctx->st));
+    k = 0;
+    for (int p = 0; p < N; ++p)
+        for (int q = 0; q <= p; ++q, ++k) h[(size_t)p * N + q] = h[(size_t)q * N + p] = v[k];
+    if (diag) memcpy(diag, h.data(), sizeof(double) * nn);
+    return set_schwarz(ctx, h);
+}
+
+int mmd_set_schwarz(mmd_ctx* ctx, const double* diag)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (ctx->nbf == 0) return fail(MMD_ERR_STATE, "no basis set");
+    if (!diag) return fail(MMD_ERR_ARG, "null pointer");
+    std::vector<double> h(diag, diag + (size_t)ctx->nbf * ctx->nbf);
+    return set_schwarz(ctx, h);
+}
+
+int mmd_eri_tensor(mmd_ctx* ctx, double* out, int out_on_device)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!out) return fail(MMD_ERR_ARG, "null output");
+    if (ctx->nbf == 0) return fail(MMD_ERR_STATE, "no basis set");
+    const size_t n4 = (size_t)ctx->nbf * ctx->nbf * ctx->nbf * ctx->nbf;
+    DevBuf<double> tmp;
+    double* dst = out;
+    if (!out_on_device) {
+        HIP_TRY(tmp.reserve(n4));
+        dst = tmp.p;
+    }
+    HIP_TRY(launch_canonical_quartets(0, ctx->nbf, ctx->d_bf.p, ctx->d_bf_exp.p, ctx->d_bf_w.p, ctx->d_boys.p, nullptr,
+                                      nullptr, nullptr, 0.0, dst, ctx->st));
+    if (!out_on_device) HIP_TRY(hipMemcpyAsync(out, dst, sizeof(double) * n4, hipMemcpyDeviceToHost, ctx->st));
+    HIP_TRY(hipStreamSynchronize(ctx->st));
+    return MMD_OK;
+}
+
+int mmd_eri_list(mmd_ctx* ctx, int64_t n, const int64_t* ijkl, double* out)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (ctx->nbf == 0) return fail(MMD_ERR_STATE, "no basis set");
+    if (n < 0 || (n > 0 && (!ijkl || !out))) return fail(MMD_ERR_ARG, "invalid quartet list");
+    if (n == 0) return MMD_OK;
+    for (int64_t i = 0; i < 4 * n; ++i)
+        if (ijkl[i] < 0 || ijkl[i] >= ctx->nbf) return fail(MMD_ERR_ARG, "basis-function index out of range");
+    DevBuf<int64_t> d_idx;
+    DevBuf<double> d_out;
+    HIP_TRY(d_idx.reserve(4 * n));
+    HIP_TRY(d_out.reserve(n));
+    HIP_TRY(hipMemcpyAsync(d_idx.p, ijkl, sizeof(int64_t) * 4 * n, hipMemcpyHostToDevice, ctx->st));
+    HIP_TRY(launch_eri_bf_list(n, d_idx.p, ctx->d_bf.p, ctx->d_bf_exp.p, ctx->d_bf_w.p, ctx->d_boys.p, d_out.p, ctx->st));
+    HIP_TRY(hipMemcpyAsync(out, d_out.p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->st));
+    HIP_TRY(hipStreamSynchronize(ctx->st));
+    return MMD_OK;
+}
+
+int mmd_fock_delta(mmd_ctx* ctx, const double* dP_re, const double* dP_im, double tol, int raw, double* G_re,
+                   double* G_im, int on_device, int64_t* nsurv)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!ctx->schwarz_ok) return fail(MMD_ERR_STATE, "mmd_schwarz must be called before mmd_fock_delta");
+    if (!dP_re || !G_re || (dP_im && !G_im)) return fail(MMD_ERR_ARG, "null pointer");
+    if (!raw && dP_im) return fail(MMD_ERR_ARG, "the symmetrised build needs a real density difference");
+    if (!raw && !ctx->pairs_ok) return fail(MMD_ERR_UNSUPPORTED, "%s", ctx->shells_err.c_str());
+    const int N = ctx->nbf;
+    const size_t nn = (size_t)N * N;
+    const hipMemcpyKind k_in = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const hipMemcpyKind k_out = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    HIP_TRY(hipEventRecord(ctx->ev0, ctx->st));
+    HIP_TRY(hipMemcpyAsync(ctx->d_D.p, dP_re, sizeof(double) * nn, k_in, ctx->st));
+    if (dP_im) HIP_TRY(hipMemcpyAsync(ctx->d_Dim.p, dP_im, sizeof(double) * nn, k_in, ctx->st));
+    HIP_TRY(launch_abs_density((int)nn, ctx->d_D.p, dP_im ? ctx->d_Dim.p : nullptr, ctx->d_absD.p, ctx->st));
+    if (!raw) {
+        HIP_TRY(launch_shell_block_max((int)ctx->shells.size(), ctx->d_sh_bf0.p, ctx->d_sh_nbf.p, N, ctx->d_absD.p,
+                                       ctx->d_Dsh.p, ctx->st));
+        rc = screen_all(ctx, tol, 0);
+        if (rc) return rc;
+        ClassArgs A = base_args(ctx);
+        A.absD = ctx->d_absD.p;
+        A.sqS = ctx->d_sqS.p;
+        A.tol = tol;
+        A.D = ctx->d_D.p;
+        A.G = ctx->d_G.p;
+        HIP_TRY(launch_fill(nn, ctx->d_G.p, 0.0, ctx->st));
+        rc = run_fock_classes(ctx, A);
+        if (rc) return rc;
+        HIP_TRY(launch_symmetrize(N, ctx->d_G.p, ctx->d_Gout.p, ctx->st));
+        HIP_TRY(hipMemcpyAsync(G_re, ctx->d_Gout.p, sizeof(double) * nn, k_out, ctx->st));
+    } else {
+        for (int part = 0; part < (dP_im ? 2 : 1); ++part) {
+            HIP_TRY(launch_fill(nn, ctx->d_G.p, 0.0, ctx->st));
+            HIP_TRY(launch_canonical_quartets(1, N, ctx->d_bf.p, ctx->d_bf_exp.p, ctx->d_bf_w.p, ctx->d_boys.p,
+                                              part == 0 ? ctx->d_D.p : ctx->d_Dim.p, ctx->d_absD.p, ctx->d_sqS.p, tol,
+                                              ctx->d_G.p, ctx->st));
+            HIP_TRY(hipMemcpyAsync(part == 0 ? G_re : G_im, ctx->d_G.p, sizeof(double) * nn, k_out, ctx->st));
+        }
+    }
+    HIP_TRY(hipEventRecord(ctx->ev1, ctx->st));
+    if (nsurv && !raw) {
+        int h[NQCLASS];
+        HIP_TRY(hipMemcpyAsync(h, ctx->d_count.p, sizeof(int) * NQCLASS, hipMemcpyDeviceToHost, ctx->st));
+        HIP_TRY(hipStreamSynchronize(ctx->st));
+        int64_t s = 0;
+        for (int i = 0; i < NQCLASS; ++i) s += h[i];
+        *nsurv = s;
+    } else if (nsurv) {
+        *nsurv = -1;
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->st));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_fock_ms = ms;
+    return MMD_OK;
+}
+
+int mmd_jk_tensor(mmd_ctx* ctx, const double* T, int t_on_device, const double* P_re, const double* P_im,
+                  double* J_re, double* J_im, double* K_re, double* K_im)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (ctx->nbf == 0) return fail(MMD_ERR_STATE, "no basis set");
+    if (!T || !P_re || !J_re || !K_re || (P_im && (!J_im || !K_im))) return fail(MMD_ERR_ARG, "null pointer");
+    const int N = ctx->nbf;
+    const size_t nn = (size_t)N * N, n4 = nn * nn;
+    DevBuf<double> dT, d;
+    const double* t = T;
+    if (!t_on_device) {
+        HIP_TRY(dT.reserve(n4));
+        HIP_TRY(hipMemcpyAsync(dT.p, T, sizeof(double) * n4, hipMemcpyHostToDevice, ctx->st));
+        t = dT.p;
+    }
+    HIP_TRY(d.reserve(6 * nn));
+    HIP_TRY(hipMemcpyAsync(d.p, P_re, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->st));
+    if (P_im) HIP_TRY(hipMemcpyAsync(d.p + nn, P_im, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->st));
+    HIP_TRY(launch_jk_tensor(N, t, d.p, P_im ? d.p + nn : nullptr, d.p + 2 * nn, d.p + 3 * nn, d.p + 4 * nn,
+                             d.p + 5 * nn, ctx->st));
+    HIP_TRY(hipMemcpyAsync(J_re, d.p + 2 * nn, sizeof(double) * nn, hipMemcpyDeviceToHost, ctx->st));
+    HIP_TRY(hipMemcpyAsync(K_re, d.p + 4 * nn, sizeof(double) * nn, hipMemcpyDeviceToHost, ctx->st));
+    if (P_im) {
+        HIP_TRY(hipMemcpyAsync(J_im, d.p + 3 * nn, sizeof(double) * nn, hipMemcpyDeviceToHost, ctx->st));
+        HIP_TRY(hipMemcpyAsync(K_im, d.p + 5 * nn, sizeof(double) * nn, hipMemcpyDeviceToHost, ctx->st));
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->st));
+    return MMD_OK;
+}
+
+int mmd_boys(mmd_ctx* ctx, int n, const double* m, const double* T, int table, double* out)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && (!m || !T || !out))) return fail(MMD_ERR_ARG, "invalid arguments");
+    for (int i = 0; i < n; ++i) {
+        if (!(m[i] >= 0.0) || !(T[i] >= 0.0)) return fail(MMD_ERR_ARG, "Boys arguments must be non-negative");
+        if (table && (m[i] != floor(m[i]) || m[i] > MAX_LTOT))
+            return fail(MMD_ERR_ARG, "tabulated Boys path takes integer m <= %d", MAX_LTOT);
+    }
+    if (n == 0) return MMD_OK;
+    DevBuf<double> d;
+    HIP_TRY(d.reserve(3 * (size_t)n));
+    HIP_TRY(hipMemcpyAsync(d.p, m, sizeof(double) * n, hipMemcpyHostToDevice, ctx->st));
+    HIP_TRY(hipMemcpyAsync(d.p + n, T, sizeof(double) * n, hipMemcpyHostToDevice, ctx->st));
+    HIP_TRY(launch_boys_eval(n, d.p, d.p + n, table, ctx->d_boys.p, d.p + 2 * n, ctx->st));
+    HIP_TRY(hipMemcpyAsync(out, d.p + 2 * n, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->st));
+    HIP_TRY(hipStreamSynchronize(ctx->st));
+    return MMD_OK;
+}
+
+int mmd_last_fock_ms(mmd_ctx* ctx, double* ms)
+{
+    if (!ctx || !ms) return fail(MMD_ERR_ARG, "null pointer");
+    *ms = ctx->last_fock_ms;
+    return MMD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,66 @@
+// Kernel entry points of libmmd (host-side declarations).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <vector>
+#include "mmd_types.h"
+
+namespace mmd {
+
+// Quartet class index: qc = pcb (pcb+1)/2 + pck with pcb >= pck.
+inline __host__ __device__ void qclass_decode(int qc, int* la, int* lb, int* lc, int* ld)
+{
+    int pcb = 0;
+    while ((pcb + 1) * (pcb + 2) / 2 <= qc) ++pcb;
+    int pck = qc - pcb * (pcb + 1) / 2;
+    int l1 = 0;
+    while ((l1 + 1) * (l1 + 2) / 2 <= pcb) ++l1;
+    *la = l1;
+    *lb = pcb - l1 * (l1 + 1) / 2;
+    int l3 = 0;
+    while ((l3 + 1) * (l3 + 2) / 2 <= pck) ++l3;
+    *lc = l3;
+    *ld = pck - l3 * (l3 + 1) / 2;
+}
+
+hipError_t launch_fock_class(int qc, const ClassArgs& A, int grid, hipStream_t st);
+int fock_class_nchunk(int qc);
+
+// A single Cartesian basis function for the generic (per basis-function
+// quartet) kernel: centre, exponents (l,m,n), primitive range; the
+// primitive weights are norm*coef products as cython/basis.pxi stores them.
+struct BFDesc {
+    double x, y, z;
+    int32_t l, m, n;
+    int32_t p0, np;
+    int32_t pad;
+};
+
+hipError_t launch_eri_bf_list(int64_t nq, const int64_t* ijkl, const BFDesc* bfs, const double* pexp,
+                              const double* pw, const double* boys, double* out, hipStream_t st);
+hipError_t launch_canonical_quartets(int mode, int N, const BFDesc* bfs, const double* pexp, const double* pw,
+                                     const double* boys, const double* D, const double* absD, const double* sqS,
+                                     double tol, double* out, hipStream_t st);
+hipError_t launch_one_electron(int nbf, const BFDesc* bfs, const double* pexp, const double* pw, int natom,
+                               const double* atoms /* natom x 4: Z, x, y, z */, double cx, double cy, double cz,
+                               const double* boys, double* S, double* T, double* V, double* M, double* Lm,
+                               hipStream_t st);
+hipError_t launch_screen_class(int boff, int nb, int koff, int nk, int tri, const ShellPair* pairs,
+                               const double* Dsh, int nshell, double tol, int keep_all, int2* surv, int* count,
+                               hipStream_t st);
+hipError_t launch_abs_density(int n, const double* Dre, const double* Dim, double* absD, hipStream_t st);
+hipError_t launch_shell_block_max(int nshell, const int* sh_bf0, const int* sh_nbf, int N, const double* absD,
+                                  double* Dsh, hipStream_t st);
+hipError_t launch_fill(size_t n, double* x, double v, hipStream_t st);
+hipError_t launch_fill_int(int n, int* x, int v, hipStream_t st);
+hipError_t launch_symmetrize(int N, const double* G, double* out, hipStream_t st);
+hipError_t launch_sqrt_clamped(size_t n, const double* x, double* y, hipStream_t st);
+hipError_t launch_jk_tensor(int N, const double* T, const double* Pre, const double* Pim, double* Jre,
+                            double* Jim, double* Kre, double* Kim, hipStream_t st);
+
+}  // namespace mmd
+
+namespace mmd {
+void boys_build_table(std::vector<double>& tab);
+hipError_t launch_boys_eval(int n, const double* m, const double* T, int table, const double* tab, double* out,
+                            hipStream_t st);
+}  // namespace mmd

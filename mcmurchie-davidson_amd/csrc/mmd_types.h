@@ -1,0 +1,47 @@
+// Data layout shared by the host runtime and the gfx950 kernels.
+#pragma once
+#include <stdint.h>
+
+namespace mmd {
+
+// A contracted shell pair in canonical orientation (la >= lb).  Shell pairs
+// are grouped by angular-momentum pair class and, inside a class, sorted by
+// primitive-pair count and Schwarz factor (descending) so that the quartets a
+// wavefront enumerates have similar cost and similar screening fate.
+struct ShellPair {
+    int32_t sa, sb;      // shell indices (sa carries la >= lb)
+    int32_t bfa, bfb;    // first basis-function index of each shell
+    int32_t pp0, npp;    // range in the primitive-pair table
+    int32_t same;        // sa == sb
+    int32_t pad;
+    double q;            // max over the pair's functions of sqrt((ab|ab))
+};
+
+// A primitive pair: exponent sum, Gaussian-product prefactor including both
+// contraction coefficients, product centre and its offsets from A and B.
+struct PrimPair {
+    double p, K;
+    double Px, Py, Pz;
+    double PAx, PAy, PAz;
+    double PBx, PBy, PBz;
+    double inv2p, pad;
+};
+
+// Arguments of the quartet-class kernels (passed by value).
+struct ClassArgs {
+    const int2* quart;          // surviving (bra pair, ket pair) for this class
+    const int* count;           // number of survivors (device)
+    const ShellPair* pairs;     // all shell pairs
+    const PrimPair* pp;         // primitive pairs
+    const double* boys;         // Boys table [BOYS_ROWS][BOYS_NT]
+    const double* gcomp;        // per-basis-function component scale
+    const double* D;            // density (difference) N x N
+    const double* absD;         // |dP| N x N (screening)
+    const double* sqS;          // sqrt((ij|ij)) N x N (screening)
+    double tol;                 // screening threshold
+    int N;
+    double* G;                  // Fock accumulation N x N
+};
+
+
+}  // namespace mmd

@@ -1,0 +1,486 @@
+// Support kernels of libmmd for gfx950: screening/compaction of shell
+// quartets, density preprocessing, symmetrisation, the generic per
+// basis-function quartet ERI kernel (the ERI(a,b,c,d) entry point), the
+// one-electron integrals and the stored-tensor J/K contraction.
+#include <hip/hip_runtime.h>
+#include "mmd_device.h"
+#include "mmd_kernels.h"
+
+namespace mmd {
+
+// ---------------------------------------------------------------------------
+// Generic contracted ERI of one basis-function quartet (cython/twoe.pyx:34-94)
+// with runtime angular momenta (l <= 2 per function).  Used by the ERI(a,b,c,d)
+// entry point, the Schwarz diagonal, the stored tensor (doERIs) and formPT's
+// unsymmetrised output; the Fock build itself uses the class kernels.
+// ---------------------------------------------------------------------------
+__device__ double eri_bf_generic(const BFDesc& a, const BFDesc& b, const BFDesc& c, const BFDesc& d,
+                                 const double* __restrict__ pexp, const double* __restrict__ pw,
+                                 const double* __restrict__ boys)
+{
+    const int L = a.l + a.m + a.n + b.l + b.m + b.n + c.l + c.m + c.n + d.l + d.m + d.n;
+    const double A[3] = {a.x, a.y, a.z}, B[3] = {b.x, b.y, b.z}, C[3] = {c.x, c.y, c.z}, Dc[3] = {d.x, d.y, d.z};
+    const int la[3] = {a.l, a.m, a.n}, lb[3] = {b.l, b.m, b.n}, lc[3] = {c.l, c.m, c.n}, ld[3] = {d.l, d.m, d.n};
+    const int nbx = la[0] + lb[0] + 1, nby = la[1] + lb[1] + 1, nbz = la[2] + lb[2] + 1;
+    const int nkx = lc[0] + ld[0] + 1, nky = lc[1] + ld[1] + 1, nkz = lc[2] + ld[2] + 1;
+    double total = 0.0;
+    for (int ia = 0; ia < a.np; ++ia)
+        for (int ib = 0; ib < b.np; ++ib) {
+            const double ea = pexp[a.p0 + ia], eb = pexp[b.p0 + ib];
+            const double p = ea + eb;
+            double P[3], Eb[3][45], kab = 1.0;
+            for (int x = 0; x < 3; ++x) {
+                P[x] = (ea * A[x] + eb * B[x]) / p;
+                kab *= exp(-ea * eb / p * (A[x] - B[x]) * (A[x] - B[x]));
+                etab_rt(la[x], lb[x], 0.5 / p, P[x] - A[x], P[x] - B[x], Eb[x]);
+            }
+            const double wab = pw[a.p0 + ia] * pw[b.p0 + ib] * kab;
+            for (int ic = 0; ic < c.np; ++ic)
+                for (int id = 0; id < d.np; ++id) {
+                    const double ec = pexp[c.p0 + ic], ed = pexp[d.p0 + id];
+                    const double qq = ec + ed;
+                    double Q[3], Ek[3][45], kcd = 1.0;
+                    for (int x = 0; x < 3; ++x) {
+                        Q[x] = (ec * C[x] + ed * Dc[x]) / qq;
+                        kcd *= exp(-ec * ed / qq * (C[x] - Dc[x]) * (C[x] - Dc[x]));
+                        etab_rt(lc[x], ld[x], 0.5 / qq, Q[x] - C[x], Q[x] - Dc[x], Ek[x]);
+                    }
+                    const double wcd = pw[c.p0 + ic] * pw[d.p0 + id] * kcd;
+                    const double alpha = p * qq / (p + qq);
+                    const double X = P[0] - Q[0], Y = P[1] - Q[1], Z = P[2] - Q[2];
+                    const double T = alpha * (X * X + Y * Y + Z * Z);
+                    double F[MAX_LTOT + 1];
+                    boys_runtime(L, T, boys, F);
+                    double s = 1.0;
+                    for (int m = 0; m <= L; ++m) { F[m] *= s; s *= -2.0 * alpha; }
+                    double R[nherm(MAX_LTOT)];
+                    hermite_R_rt(L, X, Y, Z, F, R);
+                    double val = 0.0;
+                    for (int t = 0; t < nbx; ++t)
+                        for (int u = 0; u < nby; ++u)
+                            for (int v = 0; v < nbz; ++v) {
+                                const double eb3 = Eb[0][(la[0] * (lb[0] + 1) + lb[0]) * nbx + t]
+                                                 * Eb[1][(la[1] * (lb[1] + 1) + lb[1]) * nby + u]
+                                                 * Eb[2][(la[2] * (lb[2] + 1) + lb[2]) * nbz + v];
+                                for (int tau = 0; tau < nkx; ++tau)
+                                    for (int nu = 0; nu < nky; ++nu)
+                                        for (int phi = 0; phi < nkz; ++phi) {
+                                            const double sg = ((tau + nu + phi) & 1) ? -1.0 : 1.0;
+                                            val += eb3 * sg
+                                                 * Ek[0][(lc[0] * (ld[0] + 1) + ld[0]) * nkx + tau]
+                                                 * Ek[1][(lc[1] * (ld[1] + 1) + ld[1]) * nky + nu]
+                                                 * Ek[2][(lc[2] * (ld[2] + 1) + ld[2]) * nkz + phi]
+                                                 * R[hidx(t + tau, u + nu, v + phi)];
+                                        }
+                            }
+                    total += val * 34.98683665524972497 / (p * qq * sqrt(p + qq)) * wab * wcd;
+                }
+        }
+    return total;
+}
+
+__global__ void __launch_bounds__(256) eri_bf_list_kernel(int64_t nq, const int64_t* __restrict__ ijkl,
+                                                          const BFDesc* __restrict__ bfs,
+                                                          const double* __restrict__ pexp,
+                                                          const double* __restrict__ pw,
+                                                          const double* __restrict__ boys, double* out)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    out[q] = eri_bf_generic(bfs[ijkl[4 * q + 0]], bfs[ijkl[4 * q + 1]], bfs[ijkl[4 * q + 2]], bfs[ijkl[4 * q + 3]],
+                            pexp, pw, boys);
+}
+
+hipError_t launch_eri_bf_list(int64_t nq, const int64_t* ijkl, const BFDesc* bfs, const double* pexp,
+                              const double* pw, const double* boys, double* out, hipStream_t st)
+{
+    if (nq <= 0) return hipSuccess;
+    const int64_t grid = (nq + 255) / 256;
+    hipLaunchKernelGGL(eri_bf_list_kernel, dim3((unsigned)grid), dim3(256), 0, st, nq, ijkl, bfs, pexp, pw, boys,
+                       out);
+    return hipGetLastError();
+}
+
+// Triangular index -> (r, c), r >= c.
+__device__ inline void tri_decode(int64_t idx, int64_t* r, int64_t* c)
+{
+    int64_t x = (int64_t)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
+    while (x * (x + 1) / 2 > idx) --x;
+    while ((x + 1) * (x + 2) / 2 <= idx) ++x;
+    *r = x;
+    *c = idx - x * (x + 1) / 2;
+}
+
+// Canonical basis-function quartets i>=j, k>=l, ij>=kl (cython/twoe.pyx:15-21,
+// cython/fock.pyx:38-44), enumerated as a triangle of pair indices.
+//   mode 0: store (ij|kl) at all 8 permutations (doERIs, cython/twoe.pyx:23-30)
+//   mode 1: formPT's unsymmetrised update (cython/fock.pyx:46-85)
+__global__ void __launch_bounds__(256) canonical_quartet_kernel(int mode, int N, const BFDesc* __restrict__ bfs,
+                                                                const double* __restrict__ pexp,
+                                                                const double* __restrict__ pw,
+                                                                const double* __restrict__ boys,
+                                                                const double* __restrict__ D,
+                                                                const double* __restrict__ absD,
+                                                                const double* __restrict__ sqS, double tol,
+                                                                double* __restrict__ out)
+{
+    const int64_t npair = (int64_t)N * (N + 1) / 2;
+    const int64_t total = npair * (npair + 1) / 2;
+    const size_t n = (size_t)N;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        int64_t ij, kl, i, j, k, l;
+        tri_decode(idx, &ij, &kl);
+        tri_decode(ij, &i, &j);
+        tri_decode(kl, &k, &l);
+        if (mode == 0) {
+            const double v = eri_bf_generic(bfs[i], bfs[j], bfs[k], bfs[l], pexp, pw, boys);
+            out[((i * n + j) * n + k) * n + l] = v;
+            out[((k * n + l) * n + i) * n + j] = v;
+            out[((j * n + i) * n + l) * n + k] = v;
+            out[((l * n + k) * n + j) * n + i] = v;
+            out[((j * n + i) * n + k) * n + l] = v;
+            out[((l * n + k) * n + i) * n + j] = v;
+            out[((i * n + j) * n + l) * n + k] = v;
+            out[((k * n + l) * n + j) * n + i] = v;
+        } else {
+            const double bound = sqS[i * n + j] * sqS[k * n + l]
+                * fmax(fmax(fmax(4.0 * absD[i * n + j], 4.0 * absD[k * n + l]), fmax(absD[i * n + k], absD[i * n + l])),
+                       fmax(absD[j * n + k], absD[j * n + l]));
+            if (bound < tol) continue;
+            const double s12 = (i == j) ? 1.0 : 2.0;
+            const double s34 = (k == l) ? 1.0 : 2.0;
+            const double s1234 = (i == k && j == l) ? 1.0 : 2.0;
+            const double eri = s12 * s34 * s1234 * eri_bf_generic(bfs[i], bfs[j], bfs[k], bfs[l], pexp, pw, boys);
+            atomicAdd(&out[i * n + j], D[k * n + l] * eri);
+            atomicAdd(&out[k * n + l], D[i * n + j] * eri);
+            atomicAdd(&out[i * n + k], -0.25 * D[j * n + l] * eri);
+            atomicAdd(&out[j * n + l], -0.25 * D[i * n + k] * eri);
+            atomicAdd(&out[i * n + l], -0.25 * D[j * n + k] * eri);
+            atomicAdd(&out[k * n + j], -0.25 * D[i * n + l] * eri);
+        }
+    }
+}
+
+hipError_t launch_canonical_quartets(int mode, int N, const BFDesc* bfs, const double* pexp, const double* pw,
+                                     const double* boys, const double* D, const double* absD, const double* sqS,
+                                     double tol, double* out, hipStream_t st)
+{
+    const int64_t npair = (int64_t)N * (N + 1) / 2;
+    const int64_t total = npair * (npair + 1) / 2;
+    if (total <= 0) return hipSuccess;
+    int64_t grid = (total + 255) / 256;
+    if (grid > 65536) grid = 65536;
+    hipLaunchKernelGGL(canonical_quartet_kernel, dim3((unsigned)grid), dim3(256), 0, st, mode, N, bfs, pexp, pw, boys,
+                       D, absD, sqS, tol, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// One-electron integrals (cython/onee.pyx:14-192, assembled as
+// mmd/molecule.py:253-276).  One lane per basis-function pair i >= j.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) one_electron_kernel(int nbf, const BFDesc* __restrict__ bfs,
+                                                           const double* __restrict__ pexp,
+                                                           const double* __restrict__ pw, int natom,
+                                                           const double* __restrict__ atoms, double cx, double cy,
+                                                           double cz, const double* __restrict__ boys, double* S,
+                                                           double* T, double* V, double* M, double* Lm)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t npair = (int64_t)nbf * (nbf + 1) / 2;
+    if (idx >= npair) return;
+    int i = (int)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
+    while ((int64_t)i * (i + 1) / 2 > idx) --i;
+    while ((int64_t)(i + 1) * (i + 2) / 2 <= idx) ++i;
+    const int j = (int)(idx - (int64_t)i * (i + 1) / 2);
+    const BFDesc a = bfs[i], b = bfs[j];
+    const double A[3] = {a.x, a.y, a.z}, B[3] = {b.x, b.y, b.z}, Cc[3] = {cx, cy, cz};
+    const int l1[3] = {a.l, a.m, a.n}, l2[3] = {b.l, b.m, b.n};
+    double s = 0, t = 0, v = 0, mu[3] = {0, 0, 0}, lam[3] = {0, 0, 0};
+    for (int ia = 0; ia < a.np; ++ia)
+        for (int ib = 0; ib < b.np; ++ib) {
+            const double ea = pexp[a.p0 + ia], eb = pexp[b.p0 + ib];
+            const double p = ea + eb;
+            const double w = pw[a.p0 + ia] * pw[b.p0 + ib];
+            double P[3], kab = 1.0;
+            // E tables up to (l1+1, l2+2) per dimension: 4 x 5 x 8 entries.
+            double E[3][160];
+            int LI[3], LJ[3], NT[3];
+            for (int x = 0; x < 3; ++x) {
+                P[x] = (ea * A[x] + eb * B[x]) / p;
+                kab *= exp(-ea * eb / p * (A[x] - B[x]) * (A[x] - B[x]));
+                LI[x] = l1[x] + 1;
+                LJ[x] = l2[x] + 2;
+                NT[x] = LI[x] + LJ[x] + 1;
+                etab_rt(LI[x], LJ[x], 0.5 / p, P[x] - A[x], P[x] - B[x], E[x]);
+            }
+#define E1(x, ii, jj, tt) \
+    (((jj) < 0 || (tt) > (ii) + (jj)) ? 0.0 : E[x][((ii) * (LJ[x] + 1) + (jj)) * NT[x] + (tt)])
+            const double pre = pow(3.141592653589793238462643383279 / p, 1.5) * kab * w;
+            double S0[3], S1[3], D1[3], Tk[3], Dp[3];
+            for (int x = 0; x < 3; ++x) {
+                const int ii = l1[x], jj = l2[x];
+                S0[x] = E1(x, ii, jj, 0);
+                // kinetic pieces (cython/onee.pyx:109-136)
+                Tk[x] = (2.0 * jj + 1.0) * eb * E1(x, ii, jj, 0) - 2.0 * eb * eb * E1(x, ii, jj + 2, 0)
+                      - 0.5 * jj * (jj - 1.0) * E1(x, ii, jj - 2, 0);
+                // dipole (cython/onee.pyx:81-107)
+                Dp[x] = E1(x, ii, jj, 1) + (P[x] - Cc[x]) * E1(x, ii, jj, 0);
+                // angular momentum (cython/onee.pyx:139-173)
+                S1[x] = E1(x, ii + 1, jj, 0) + (A[x] - Cc[x]) * E1(x, ii, jj, 0);
+                D1[x] = jj * E1(x, ii, jj - 1, 0) - 2.0 * eb * E1(x, ii, jj + 1, 0);
+            }
+            s += S0[0] * S0[1] * S0[2] * pre;
+            t += (Tk[0] * S0[1] * S0[2] + Tk[1] * S0[0] * S0[2] + Tk[2] * S0[0] * S0[1]) * pre;
+            mu[0] += Dp[0] * S0[1] * S0[2] * pre;
+            mu[1] += S0[0] * Dp[1] * S0[2] * pre;
+            mu[2] += S0[0] * S0[1] * Dp[2] * pre;
+            lam[0] += -S0[0] * (S1[1] * D1[2] - S1[2] * D1[1]) * pre;
+            lam[1] += -S0[1] * (S1[2] * D1[0] - S1[0] * D1[2]) * pre;
+            lam[2] += -S0[2] * (S1[0] * D1[1] - S1[1] * D1[0]) * pre;
+            // nuclear attraction (cython/onee.pyx:175-192), summed over atoms
+            const int Lv = l1[0] + l2[0] + l1[1] + l2[1] + l1[2] + l2[2];
+            for (int at = 0; at < natom; ++at) {
+                const double Z = atoms[4 * at];
+                const double X = P[0] - atoms[4 * at + 1], Y = P[1] - atoms[4 * at + 2], W = P[2] - atoms[4 * at + 3];
+                double F[2 * MAX_L + 1];
+                boys_runtime(Lv, p * (X * X + Y * Y + W * W), boys, F);
+                double sc = 1.0;
+                for (int m = 0; m <= Lv; ++m) { F[m] *= sc; sc *= -2.0 * p; }
+                double R[nherm(2 * MAX_L)];
+                hermite_R_rt(Lv, X, Y, W, F, R);
+                double val = 0.0;
+                for (int tt = 0; tt <= l1[0] + l2[0]; ++tt)
+                    for (int uu = 0; uu <= l1[1] + l2[1]; ++uu)
+                        for (int vv = 0; vv <= l1[2] + l2[2]; ++vv)
+                            val += E1(0, l1[0], l2[0], tt) * E1(1, l1[1], l2[1], uu) * E1(2, l1[2], l2[2], vv)
+                                 * R[hidx(tt, uu, vv)];
+                v += -Z * val * 2.0 * 3.141592653589793238462643383279 / p * kab * w;
+            }
+#undef E1
+        }
+    const size_t n = (size_t)nbf;
+    S[i * n + j] = S[j * n + i] = s;
+    T[i * n + j] = T[j * n + i] = t;
+    V[i * n + j] = V[j * n + i] = v;
+    for (int x = 0; x < 3; ++x) {
+        M[x * n * n + i * n + j] = M[x * n * n + j * n + i] = mu[x];
+        Lm[x * n * n + i * n + j] = lam[x];
+        Lm[x * n * n + j * n + i] = -lam[x];
+    }
+}
+
+hipError_t launch_one_electron(int nbf, const BFDesc* bfs, const double* pexp, const double* pw, int natom,
+                               const double* atoms, double cx, double cy, double cz, const double* boys, double* S,
+                               double* T, double* V, double* M, double* Lm, hipStream_t st)
+{
+    const int64_t npair = (int64_t)nbf * (nbf + 1) / 2;
+    if (npair <= 0) return hipSuccess;
+    hipLaunchKernelGGL(one_electron_kernel, dim3((unsigned)((npair + 255) / 256)), dim3(256), 0, st, nbf, bfs, pexp,
+                       pw, natom, atoms, cx, cy, cz, boys, S, T, V, M, Lm);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Shell-quartet screening and compaction for one quartet class.  Quartets are
+// enumerated implicitly: bra pair P of class b (local index < nb), ket pair Q
+// of class k (< nk); P >= Q when both classes are the same (tri).  A quartet
+// survives if q_P q_Q max(4 D_ab, 4 D_cd, D_ac, D_ad, D_bc, D_bd) >= tol with
+// D the shell-block maxima of |dP| -- an upper bound of the per
+// basis-function test of cython/fock.pyx:46-57.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) screen_class_kernel(int boff, int nb, int koff, int nk, int tri,
+                                                           const ShellPair* __restrict__ pairs,
+                                                           const double* __restrict__ Dsh, int nshell, double tol,
+                                                           int keep_all, int2* __restrict__ surv, int* count)
+{
+    const int64_t total = tri ? (int64_t)nb * (nb + 1) / 2 : (int64_t)nb * nk;
+    const int lane = threadIdx.x & 63;
+    const int64_t wave0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t base = wave0 * 64; base < total; base += nwaves * 64) {
+        const int64_t idx = base + lane;
+        bool keep = false;
+        int P = 0, Q = 0;
+        if (idx < total) {
+            if (tri) {
+                int r = (int)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
+                while ((int64_t)r * (r + 1) / 2 > idx) --r;
+                while ((int64_t)(r + 1) * (r + 2) / 2 <= idx) ++r;
+                P = r;
+                Q = (int)(idx - (int64_t)r * (r + 1) / 2);
+            } else {
+                P = (int)(idx / nk);
+                Q = (int)(idx % nk);
+            }
+            if (keep_all) {
+                keep = true;
+            } else {
+                const ShellPair bra = pairs[boff + P], ket = pairs[koff + Q];
+                const size_t ns = (size_t)nshell;
+                const double dm = fmax(fmax(fmax(4.0 * Dsh[bra.sa * ns + bra.sb], 4.0 * Dsh[ket.sa * ns + ket.sb]),
+                                            fmax(Dsh[bra.sa * ns + ket.sa], Dsh[bra.sa * ns + ket.sb])),
+                                       fmax(Dsh[bra.sb * ns + ket.sa], Dsh[bra.sb * ns + ket.sb]));
+                keep = bra.q * ket.q * dm >= tol;
+            }
+        }
+        const unsigned long long m = __ballot(keep);
+        const int cnt = __popcll(m);
+        int pos = 0;
+        if (lane == 0 && cnt) pos = atomicAdd(count, cnt);
+        pos = __shfl(pos, 0);
+        if (keep) {
+            const int off = __popcll(m & ((1ull << lane) - 1ull));
+            surv[pos + off] = make_int2(boff + P, koff + Q);
+        }
+    }
+}
+
+hipError_t launch_screen_class(int boff, int nb, int koff, int nk, int tri, const ShellPair* pairs, const double* Dsh,
+                               int nshell, double tol, int keep_all, int2* surv, int* count, hipStream_t st)
+{
+    const int64_t total = tri ? (int64_t)nb * (nb + 1) / 2 : (int64_t)nb * nk;
+    if (total <= 0) return hipSuccess;
+    int64_t grid = (total + 255) / 256;
+    if (grid > 8192) grid = 8192;
+    hipLaunchKernelGGL(screen_class_kernel, dim3((unsigned)grid), dim3(256), 0, st, boff, nb, koff, nk, tri, pairs,
+                       Dsh, nshell, tol, keep_all, surv, count);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Small elementwise kernels
+// ---------------------------------------------------------------------------
+__global__ void abs_density_kernel(int n, const double* __restrict__ Dre, const double* __restrict__ Dim,
+                                   double* __restrict__ absD)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    absD[i] = Dim ? hypot(Dre[i], Dim[i]) : fabs(Dre[i]);
+}
+
+hipError_t launch_abs_density(int n, const double* Dre, const double* Dim, double* absD, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(abs_density_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, Dre, Dim, absD);
+    return hipGetLastError();
+}
+
+__global__ void shell_block_max_kernel(int nshell, const int* __restrict__ bf0, const int* __restrict__ nbf, int N,
+                                       const double* __restrict__ absD, double* __restrict__ Dsh)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nshell * nshell) return;
+    const int s1 = idx / nshell, s2 = idx % nshell;
+    double m = 0.0;
+    for (int i = 0; i < nbf[s1]; ++i)
+        for (int j = 0; j < nbf[s2]; ++j) m = fmax(m, absD[(size_t)(bf0[s1] + i) * N + bf0[s2] + j]);
+    Dsh[idx] = m;
+}
+
+hipError_t launch_shell_block_max(int nshell, const int* sh_bf0, const int* sh_nbf, int N, const double* absD,
+                                  double* Dsh, hipStream_t st)
+{
+    const int n = nshell * nshell;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(shell_block_max_kernel, dim3((n + 255) / 256), dim3(256), 0, st, nshell, sh_bf0, sh_nbf, N,
+                       absD, Dsh);
+    return hipGetLastError();
+}
+
+__global__ void fill_kernel(size_t n, double* x, double v)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+hipError_t launch_fill(size_t n, double* x, double v, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, x, v);
+    return hipGetLastError();
+}
+
+__global__ void fill_int_kernel(int n, int* x, int v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+hipError_t launch_fill_int(int n, int* x, int v, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_int_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, x, v);
+    return hipGetLastError();
+}
+
+__global__ void symmetrize_kernel(int N, const double* __restrict__ G, double* __restrict__ out)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= N * N) return;
+    const int i = idx / N, j = idx % N;
+    out[idx] = 0.5 * (G[(size_t)i * N + j] + G[(size_t)j * N + i]);
+}
+
+hipError_t launch_symmetrize(int N, const double* G, double* out, hipStream_t st)
+{
+    if (N <= 0) return hipSuccess;
+    hipLaunchKernelGGL(symmetrize_kernel, dim3((N * N + 255) / 256), dim3(256), 0, st, N, G, out);
+    return hipGetLastError();
+}
+
+__global__ void sqrt_kernel(size_t n, const double* __restrict__ x, double* __restrict__ y)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = sqrt(x[i]);
+}
+
+hipError_t launch_sqrt_clamped(size_t n, const double* x, double* y, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(sqrt_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, x, y);
+    return hipGetLastError();
+}
+
+// Conventional J/K from the stored tensor (mmd/scf.py:97-98):
+//   J_pq = sum_rs (pq|rs) P_sr ;  K_pq = sum_rs (ps|qr) P_sr.
+__global__ void jk_tensor_kernel(int N, const double* __restrict__ T, const double* __restrict__ Pre,
+                                 const double* __restrict__ Pim, double* Jre, double* Jim, double* Kre, double* Kim)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= N * N) return;
+    const int p = idx / N, q = idx % N;
+    const size_t n = (size_t)N;
+    double jr = 0, ji = 0, kr = 0, ki = 0;
+    for (int r = 0; r < N; ++r)
+        for (int s = 0; s < N; ++s) {
+            const double tj = T[((p * n + q) * n + r) * n + s];
+            const double tk = T[((p * n + s) * n + q) * n + r];
+            const double pr = Pre[s * n + r];
+            jr += tj * pr;
+            kr += tk * pr;
+            if (Pim) {
+                const double pi = Pim[s * n + r];
+                ji += tj * pi;
+                ki += tk * pi;
+            }
+        }
+    Jre[idx] = jr;
+    Kre[idx] = kr;
+    if (Pim) {
+        Jim[idx] = ji;
+        Kim[idx] = ki;
+    }
+}
+
+hipError_t launch_jk_tensor(int N, const double* T, const double* Pre, const double* Pim, double* Jre, double* Jim,
+                            double* Kre, double* Kim, hipStream_t st)
+{
+    if (N <= 0) return hipSuccess;
+    hipLaunchKernelGGL(jk_tensor_kernel, dim3((N * N + 255) / 256), dim3(256), 0, st, N, T, Pre, Pim, Jre, Jim, Kre,
+                       Kim);
+    return hipGetLastError();
+}
+
+}  // namespace mmd

@@ -1,0 +1,273 @@
+"""ctypes binding of libmmd.so (the C ABI of include/mmd.h).
+
+The library is the only compute path of this package: every integral and
+Fock build runs in its HIP kernels.  Importing works without a GPU (so that
+host-side code can be used and tested); creating a context on a machine
+without a HIP device raises RuntimeError.  A missing library raises
+ImportError -- there is no CPU fallback.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("MMD_LIB", os.path.join(os.path.dirname(_PKG_DIR), "lib", "libmmd.so"))
+
+MMD_OK = 0
+MMD_ERR_ARG = -1
+MMD_ERR_NOMEM = -3
+MMD_ERR_DEVICE = -4
+MMD_ERR_STATE = -5
+MMD_ERR_UNSUPPORTED = -6
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        "libmmd.so not found at %s; build it with `make -C mcmurchie-davidson_amd/csrc` "
+        "(or __graft_entry__.build())" % LIB_PATH)
+
+lib = ctypes.CDLL(LIB_PATH)
+
+_c_int = ctypes.c_int
+_c_double = ctypes.c_double
+_c_i64 = ctypes.c_int64
+_vp = ctypes.c_void_p
+_ctx_p = ctypes.c_void_p
+
+lib.mmd_last_error.restype = ctypes.c_char_p
+lib.mmd_last_error.argtypes = []
+lib.mmd_version.restype = _c_int
+lib.mmd_device_count.restype = _c_int
+lib.mmd_device_count.argtypes = [ctypes.POINTER(_c_int)]
+lib.mmd_ctx_create.restype = _c_int
+lib.mmd_ctx_create.argtypes = [ctypes.POINTER(_ctx_p), _c_int]
+lib.mmd_ctx_destroy.restype = None
+lib.mmd_ctx_destroy.argtypes = [_ctx_p]
+lib.mmd_set_basis.restype = _c_int
+lib.mmd_set_basis.argtypes = [_ctx_p, _c_int, _vp, _vp, _vp, _vp, _vp]
+lib.mmd_set_atoms.restype = _c_int
+lib.mmd_set_atoms.argtypes = [_ctx_p, _c_int, _vp, _vp]
+lib.mmd_one_electron.restype = _c_int
+lib.mmd_one_electron.argtypes = [_ctx_p, _vp, _vp, _vp, _vp, _vp, _vp]
+lib.mmd_schwarz.restype = _c_int
+lib.mmd_schwarz.argtypes = [_ctx_p, _vp]
+lib.mmd_set_schwarz.restype = _c_int
+lib.mmd_set_schwarz.argtypes = [_ctx_p, _vp]
+lib.mmd_eri_tensor.restype = _c_int
+lib.mmd_eri_tensor.argtypes = [_ctx_p, _vp, _c_int]
+lib.mmd_eri_list.restype = _c_int
+lib.mmd_eri_list.argtypes = [_ctx_p, _c_i64, _vp, _vp]
+lib.mmd_fock_delta.restype = _c_int
+lib.mmd_fock_delta.argtypes = [_ctx_p, _vp, _vp, _c_double, _c_int, _vp, _vp, _c_int, ctypes.POINTER(_c_i64)]
+lib.mmd_jk_tensor.restype = _c_int
+lib.mmd_jk_tensor.argtypes = [_ctx_p, _vp, _c_int, _vp, _vp, _vp, _vp, _vp, _vp]
+lib.mmd_boys.restype = _c_int
+lib.mmd_boys.argtypes = [_ctx_p, _c_int, _vp, _vp, _c_int, _vp]
+lib.mmd_last_fock_ms.restype = _c_int
+lib.mmd_last_fock_ms.argtypes = [_ctx_p, ctypes.POINTER(_c_double)]
+
+EXPORTED = [
+    "mmd_last_error", "mmd_version", "mmd_device_count", "mmd_ctx_create", "mmd_ctx_destroy",
+    "mmd_set_basis", "mmd_set_atoms", "mmd_one_electron", "mmd_schwarz", "mmd_set_schwarz",
+    "mmd_eri_tensor", "mmd_eri_list", "mmd_fock_delta", "mmd_jk_tensor", "mmd_boys",
+    "mmd_last_fock_ms",
+]
+
+
+def check(rc):
+    """Map a libmmd status code to the reference's exception types."""
+    if rc == MMD_OK:
+        return
+    msg = lib.mmd_last_error().decode(errors="replace")
+    if rc == MMD_ERR_ARG:
+        raise ValueError(msg)
+    if rc == MMD_ERR_NOMEM:
+        raise MemoryError(msg)
+    if rc == MMD_ERR_UNSUPPORTED:
+        raise NotImplementedError(msg)
+    raise RuntimeError(msg)
+
+
+def device_count():
+    n = _c_int(0)
+    check(lib.mmd_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Context:
+    """One libmmd context (device, stream, basis, screening data)."""
+
+    def __init__(self, device=0):
+        h = _ctx_p()
+        check(lib.mmd_ctx_create(ctypes.byref(h), int(device)))
+        self._h = h
+        self.nbf = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.mmd_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- basis ---------------------------------------------------------------
+    def set_basis(self, origin, lmn, nprim, exps, weights):
+        origin = _f64(origin).reshape(-1, 3)
+        lmn = np.ascontiguousarray(lmn, dtype=np.int32).reshape(-1, 3)
+        nprim = np.ascontiguousarray(nprim, dtype=np.int32)
+        exps = _f64(exps)
+        weights = _f64(weights)
+        check(lib.mmd_set_basis(self._h, len(nprim), _ptr(origin), _ptr(lmn), _ptr(nprim),
+                                _ptr(exps), _ptr(weights)))
+        self.nbf = len(nprim)
+
+    def set_basis_from_bfs(self, bfs):
+        origin = np.array([b.origin for b in bfs], dtype=np.float64)
+        lmn = np.array([b.shell for b in bfs], dtype=np.int32)
+        nprim = np.array([b.num_exps for b in bfs], dtype=np.int32)
+        exps = np.concatenate([np.asarray(b.exps, dtype=np.float64) for b in bfs])
+        w = np.concatenate([np.asarray(b.norm, dtype=np.float64) * np.asarray(b.coefs, dtype=np.float64)
+                            for b in bfs])
+        self.set_basis(origin, lmn, nprim, exps, w)
+
+    def set_atoms(self, Z, xyz):
+        Z = _f64(Z)
+        xyz = _f64(xyz).reshape(-1, 3)
+        check(lib.mmd_set_atoms(self._h, len(Z), _ptr(Z), _ptr(xyz)))
+
+    # -- integrals -------------------------------------------------------------
+    def one_electron(self, center):
+        N = self.nbf
+        c = _f64(center)
+        S, T, V = np.empty((N, N)), np.empty((N, N)), np.empty((N, N))
+        M, L = np.empty((3, N, N)), np.empty((3, N, N))
+        check(lib.mmd_one_electron(self._h, _ptr(c), _ptr(S), _ptr(T), _ptr(V), _ptr(M), _ptr(L)))
+        return S, T, V, M, L
+
+    def schwarz(self):
+        d = np.empty((self.nbf, self.nbf))
+        check(lib.mmd_schwarz(self._h, _ptr(d)))
+        return d
+
+    def set_schwarz(self, diag):
+        d = _f64(diag)
+        check(lib.mmd_set_schwarz(self._h, _ptr(d)))
+
+    def eri_tensor(self, out=None):
+        N = self.nbf
+        if out is None:
+            out = np.empty((N, N, N, N))
+        check(lib.mmd_eri_tensor(self._h, _ptr(out), 0))
+        return out
+
+    def eri_tensor_device(self, out_ptr):
+        check(lib.mmd_eri_tensor(self._h, ctypes.c_void_p(out_ptr), 1))
+
+    def eri_list(self, ijkl):
+        ijkl = np.ascontiguousarray(ijkl, dtype=np.int64).reshape(-1, 4)
+        out = np.empty(len(ijkl))
+        check(lib.mmd_eri_list(self._h, len(ijkl), _ptr(ijkl), _ptr(out)))
+        return out
+
+    def fock_delta(self, dP_re, dP_im=None, tol=1e-12, raw=False):
+        N = self.nbf
+        dr = _f64(dP_re)
+        di = None if dP_im is None else _f64(dP_im)
+        Gr = np.empty((N, N))
+        Gi = None if di is None else np.empty((N, N))
+        ns = _c_i64(0)
+        check(lib.mmd_fock_delta(self._h, _ptr(dr), _ptr(di), float(tol), 1 if raw else 0,
+                                 _ptr(Gr), _ptr(Gi), 0, ctypes.byref(ns)))
+        return Gr, Gi, ns.value
+
+    def fock_delta_device(self, dP_ptr, G_ptr, tol=1e-12):
+        """Symmetrised G(dP) with dP and G already in device memory."""
+        ns = _c_i64(0)
+        check(lib.mmd_fock_delta(self._h, ctypes.c_void_p(dP_ptr), None, float(tol), 0,
+                                 ctypes.c_void_p(G_ptr), None, 1, ctypes.byref(ns)))
+        return ns.value
+
+    def jk_tensor(self, T, P_re, P_im=None, t_device_ptr=None):
+        N = self.nbf
+        pr = _f64(P_re)
+        pi = None if P_im is None else _f64(P_im)
+        Jr, Kr = np.empty((N, N)), np.empty((N, N))
+        Ji = Ki = None
+        if pi is not None:
+            Ji, Ki = np.empty((N, N)), np.empty((N, N))
+        if t_device_ptr is not None:
+            check(lib.mmd_jk_tensor(self._h, ctypes.c_void_p(t_device_ptr), 1, _ptr(pr), _ptr(pi),
+                                    _ptr(Jr), _ptr(Ji), _ptr(Kr), _ptr(Ki)))
+        else:
+            t = _f64(T)
+            check(lib.mmd_jk_tensor(self._h, _ptr(t), 0, _ptr(pr), _ptr(pi), _ptr(Jr), _ptr(Ji),
+                                    _ptr(Kr), _ptr(Ki)))
+        return Jr, Ji, Kr, Ki
+
+    def boys(self, m, T, table=False):
+        m = _f64(np.atleast_1d(m))
+        T = _f64(np.atleast_1d(T))
+        m, T = np.broadcast_arrays(m, T)
+        m, T = _f64(m), _f64(T)
+        out = np.empty(m.shape)
+        check(lib.mmd_boys(self._h, m.size, _ptr(m), _ptr(T), 1 if table else 0, _ptr(out)))
+        return out
+
+    def last_fock_ms(self):
+        v = _c_double(0.0)
+        check(lib.mmd_last_fock_ms(self._h, ctypes.byref(v)))
+        return v.value
+
+
+_default = {}
+_lock = threading.Lock()
+
+
+def default_context(device=0):
+    """Process-wide scratch context (for per-call entry points such as ERI())."""
+    with _lock:
+        ctx = _default.get(device)
+        if ctx is None:
+            ctx = Context(device)
+            _default[device] = ctx
+        return ctx
+
+
+# Contexts keyed by the identity of a basis-function list, so that formPT,
+# which receives only `bfs`, reuses the basis already resident on the device.
+_bfs_cache = {}
+_BFS_CACHE_MAX = 8
+
+
+def context_for_bfs(bfs, device=0):
+    key = tuple(id(b) for b in bfs)
+    with _lock:
+        hit = _bfs_cache.get(key)
+        if hit is not None:
+            return hit[0]
+    ctx = Context(device)
+    ctx.set_basis_from_bfs(bfs)
+    register_bfs_context(bfs, ctx)
+    return ctx
+
+
+def register_bfs_context(bfs, ctx):
+    key = tuple(id(b) for b in bfs)
+    with _lock:
+        if len(_bfs_cache) >= _BFS_CACHE_MAX:
+            _bfs_cache.pop(next(iter(_bfs_cache)))
+        # keep the Basis objects alive so that their ids stay unique
+        _bfs_cache[key] = (ctx, list(bfs))

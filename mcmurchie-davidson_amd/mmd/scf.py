@@ -1,0 +1,187 @@
+"""Restricted Hartree-Fock driver with the API of the reference's
+mmd/scf.py: ``RHF``, ``buildFock``, DIIS and the helpers used by the
+real-time code.
+
+The two-electron part of every Fock build runs in libmmd: the direct path
+computes the symmetrised incremental G(P - P_old) with the GPU Fock kernels
+(formPT semantics, cython/fock.pyx:10-87), the conventional path contracts
+the stored tensor on the GPU (mmd/scf.py:97-98).
+"""
+import numpy as np
+import scipy.linalg
+from numpy.linalg import multi_dot as dot
+
+from mmd.integrals.fock import formPT
+
+
+class SCF(object):
+    """SCF methods and routines for a Molecule."""
+
+    def RHF(self, doPrint=True, DIIS=True, direct=False, conver=1e-8, acc2e=1e-12):
+        """Closed-shell SCF (mmd/scf.py:10-80).  Converged when
+        ||P - P_old||_F < conver; at most 100 iterations."""
+        self.is_converged = False
+        self.delta_energy = 1e20
+        self.P_RMS = 1e20
+        self.P_old = np.zeros((self.nbasis, self.nbasis), dtype='complex')
+        self.maxiter = 100
+        self.direct = direct
+        if self.direct:
+            self.incFockRst = False
+        self.scrTol = acc2e
+        self.fock_ms = []
+        self.build(self.direct)
+
+        self.P = self.P_old
+        self.F = self.Core.astype('complex')
+
+        if DIIS:
+            self.fockSet = []
+            self.errorSet = []
+
+        for step in range(self.maxiter):
+            if step > 0:
+                self.F_old = self.F
+                energy_old = self.energy
+                self.buildFock()
+                self.P_old = self.P
+                if DIIS:
+                    F_diis = self.updateDIIS(self.F, self.P)
+                    self.FO = np.dot(self.X.T, np.dot(F_diis, self.X))
+            if not DIIS or step == 0:
+                self.orthoFock()
+
+            E, self.CO = scipy.linalg.eigh(self.FO)
+            C = np.dot(self.X, self.CO)
+            self.C = C
+            self.MO = E
+            self.P = np.dot(C[:, :self.nocc], np.conjugate(C[:, :self.nocc]).T)
+            self.computeEnergy()
+
+            if step > 0:
+                self.delta_energy = self.energy - energy_old
+                self.P_RMS = np.linalg.norm(self.P - self.P_old)
+            if np.abs(self.P_RMS) < conver or step == (self.maxiter - 1):
+                if step == (self.maxiter - 1):
+                    print("NOT CONVERGED")
+                else:
+                    self.is_converged = True
+                    FPS = np.dot(self.F, np.dot(self.P, self.S))
+                    error = FPS - self.adj(FPS)
+                    self.computeDipole()
+                    if doPrint:
+                        print("E(SCF)    = ", "{0:.12f}".format(self.energy.real) + " in " + str(step)
+                              + " iterations")
+                        print("  Convergence:")
+                        print("    FPS-SPF  = ", np.linalg.norm(error))
+                        print("    RMS(P)   = ", "{0:.2e}".format(self.P_RMS.real))
+                        print("    dE(SCF)  = ", "{0:.2e}".format(self.delta_energy.real))
+                        print("  Dipole X = ", "{0:.8f}".format(self.mu[0].real))
+                        print("  Dipole Y = ", "{0:.8f}".format(self.mu[1].real))
+                        print("  Dipole Z = ", "{0:.8f}".format(self.mu[2].real))
+                    break
+        self.iterations = step
+
+    def _direct_G(self, P, P_old):
+        """Symmetrised G(P - P_old).  A real difference takes the GPU's
+        symmetrised build; a complex one takes formPT's exact form."""
+        dP = P - P_old
+        if np.iscomplexobj(dP) and np.any(dP.imag != 0.0):
+            G = formPT(np.asarray(P, dtype=complex), np.asarray(P_old, dtype=complex), self.bfs,
+                       self.nbasis, self.screen, self.scrTol)
+            return 0.5 * (G + G.T)
+        ctx = self.native
+        if getattr(ctx, "_screen_key", None) != (id(self.screen), len(self.screen)):
+            from mmd.integrals.fock import screen_to_matrix
+            ctx.set_schwarz(screen_to_matrix(self.screen, self.nbasis))
+            ctx._screen_key = (id(self.screen), len(self.screen))
+        G, _, _ = ctx.fock_delta(np.real(dP), None, tol=self.scrTol, raw=False)
+        self.fock_ms.append(ctx.last_fock_ms())
+        return G.astype('complex')
+
+    def buildFock(self):
+        """Fock matrix from the current density (mmd/scf.py:82-100)."""
+        if self.direct:
+            if self.incFockRst:
+                self.G = self._direct_G(self.P, np.zeros_like(self.P))
+                self.F = self.Core.astype('complex') + self.G
+            else:
+                self.G = self._direct_G(self.P, self.P_old)
+                self.F = self.F_old + self.G
+        else:
+            P = np.asarray(self.P)
+            Pim = P.imag if np.iscomplexobj(P) and np.any(P.imag != 0.0) else None
+            Jr, Ji, Kr, Ki = self.native.jk_tensor(self.TwoE, np.real(P), Pim)
+            self.J = Jr.astype('complex')
+            self.K = Kr.astype('complex')
+            if Pim is not None:
+                self.J = self.J + 1j * Ji
+                self.K = self.K + 1j * Ki
+            self.G = 2. * self.J - self.K
+            self.F = self.Core.astype('complex') + self.G
+
+    def orthoFock(self):
+        self.FO = np.dot(self.X.T, np.dot(self.F, self.X))
+
+    def unOrthoFock(self):
+        self.F = np.dot(self.U.T, np.dot(self.FO, self.U))
+
+    def orthoDen(self):
+        self.PO = np.dot(self.U, np.dot(self.P, self.U.T))
+
+    def unOrthoDen(self):
+        self.P = np.dot(self.X, np.dot(self.PO, self.X.T))
+
+    def computeEnergy(self):
+        """E = sum_pq (H + F)_pq P_qp + E_nuc (mmd/scf.py:118-121)."""
+        self.el_energy = np.einsum('pq,qp', self.Core + self.F, self.P)
+        self.energy = self.el_energy + self.nuc_energy
+
+    def computeDipole(self):
+        """Dipole moment in Debye about the centre of nuclear charge
+        (mmd/scf.py:123-129)."""
+        self.el_energy = np.einsum('pq,qp', self.Core + self.F, self.P)
+        for i in range(3):
+            self.mu[i] = -2 * np.trace(np.dot(self.P, self.M[i])) + sum(
+                [atom.charge * (atom.origin[i] - self.center_of_charge[i]) for atom in self.atoms])
+        self.mu *= 2.541765
+
+    def adj(self, x):
+        assert x.shape[0] == x.shape[1]
+        return np.conjugate(x).T
+
+    def comm(self, A, B):
+        return np.dot(A, B) - np.dot(B, A)
+
+    def updateFock(self):
+        self.unOrthoDen()
+        self.buildFock()
+        self.orthoFock()
+
+    def updateDIIS(self, F, P):
+        """Pulay DIIS on the orthogonalised FPS - SPF, subspace of 8
+        (mmd/scf.py:146-180)."""
+        FPS = dot([F, P, self.S])
+        SPF = self.adj(FPS)
+        error = dot([self.X, FPS - SPF, self.X])
+        self.fockSet.append(self.F)
+        self.errorSet.append(error)
+        numFock = len(self.fockSet)
+        if numFock > 8:
+            del self.fockSet[0]
+            del self.errorSet[0]
+            numFock -= 1
+        B = np.zeros((numFock + 1, numFock + 1))
+        B[-1, :] = B[:, -1] = -1.0
+        B[-1, -1] = 0.0
+        for i in range(numFock):
+            for j in range(i + 1):
+                B[i, j] = B[j, i] = np.real(np.trace(np.dot(self.adj(self.errorSet[i]), self.errorSet[j])))
+        residual = np.zeros(numFock + 1)
+        residual[-1] = -1.0
+        weights = np.linalg.solve(B, residual)
+        assert np.isclose(sum(weights[:-1]), 1.0)
+        F = np.zeros((self.nbasis, self.nbasis), dtype='complex')
+        for i, Fock in enumerate(self.fockSet):
+            F += weights[i] * Fock
+        return F

@@ -1,0 +1,126 @@
+"""Host-side logic and the C-ABI boundary (CPU only, no GPU calls)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from geometries import METHANE, WATER, water_cluster
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BASIS = os.path.join(ROOT, "mcmurchie-davidson_amd", "mmd", "basis")
+
+
+def _gbs(name):
+    return os.path.join(BASIS, name + ".gbs")
+
+
+@pytest.mark.parametrize("geom,basis", [(WATER, "sto-3g"), (WATER, "cc-pvdz"), (METHANE, "6-31g"),
+                                        (water_cluster(3, 7.0, 2), "6-31g"), (WATER, "6-31ppgss")])
+def test_basis_order_and_normalisation_match_oracle(oracle, geom, basis):
+    from mmd.molecule import Molecule
+    mol = Molecule(geom, basis)
+    om = oracle.OracleMolecule(geom, _gbs(basis))
+    assert mol.nbasis == om.nbasis
+    assert mol.nelec == om.nelec and mol.nocc == om.nocc
+    for i, b in enumerate(mol.bfs):
+        assert np.array_equal(b.shell, om.shell[i])
+        assert np.array_equal(b.origin, om.origin[i])
+        o, k = om.poff[i], om.np[i]
+        assert b.num_exps == k
+        assert np.array_equal(b.exps, om.exps[o:o + k])
+        assert np.allclose(b.coefs, om.coefs[o:o + k], rtol=1e-14, atol=0)
+        assert np.allclose(b.norm, om.norm[o:o + k], rtol=1e-14, atol=0)
+    assert np.allclose(mol.center_of_charge, om.center_of_charge, rtol=1e-15, atol=1e-15)
+
+
+def test_cartesian_counts():
+    """Cartesian d: water cc-pVDZ has 25 functions, (H2O)8 200."""
+    from mmd.molecule import Molecule
+    assert Molecule(WATER, "cc-pvdz").nbasis == 25
+    assert Molecule(water_cluster(8, 7.0, 0), "cc-pvdz").nbasis == 200
+    assert Molecule(water_cluster(2, 7.0, 0), "6-31g").nbasis == 26
+
+
+def test_input_errors():
+    from mmd.molecule import Molecule
+    with pytest.raises(ValueError):
+        Molecule("\n0 1\nXx 0 0 0\n", "sto-3g")
+    with pytest.raises(FileNotFoundError):
+        Molecule(WATER, "no-such-basis")
+    with pytest.raises(AssertionError):
+        Molecule("\n0 1 2\nH 0 0 0\n", "sto-3g")
+    with pytest.raises(AssertionError):
+        Molecule("\n0 1\nH 0 0\n", "sto-3g")
+
+
+def test_dropin_type_errors_before_device():
+    from mmd.integrals.fock import formPT
+    from mmd.integrals.twoe import ERI, Basis, doERIs
+    b = Basis([0, 0, 0], (0, 0, 0), 1, [1.0], [1.0])
+    with pytest.raises(TypeError):
+        ERI(b, b, b, "x")
+    with pytest.raises(ValueError):
+        doERIs(1, np.zeros((1, 1, 1, 1), dtype=np.float32), [b])
+    with pytest.raises(TypeError):
+        doERIs(1, np.zeros((1, 1, 1, 1)), (b,))
+    with pytest.raises(ValueError):
+        formPT(np.zeros((1, 1)), np.zeros((1, 1), complex), [b], 1, {0: 1.0}, 1e-12)
+
+
+def test_basis_normalisation_self_overlap():
+    """S_ii = 1 in closed form for every Cartesian component (s, p, d)."""
+    from mmd.integrals.twoe import Basis
+    e = np.array([3.0, 0.9, 0.21])
+    c = np.array([0.3, 0.5, 0.4])
+    for lmn in [(0, 0, 0), (1, 0, 0), (0, 1, 0), (2, 0, 0), (1, 1, 0), (0, 1, 1)]:
+        b = Basis([0, 0, 0], lmn, 3, e, c)
+        l, m, n = lmn
+        L = l + m + n
+        df = np.prod([max(1, np.prod(np.arange(2 * x - 1, 0, -2))) for x in lmn])
+        w = b.norm * b.coefs
+        s = np.sum(np.outer(w, w) * df * np.pi ** 1.5 / 2 ** L / (e[:, None] + e[None, :]) ** (L + 1.5))
+        assert s == pytest.approx(1.0, rel=1e-14)
+
+
+def test_631g_is_derived_from_631gs(tmp_path):
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_631g", os.path.join(BASIS, "make_631g.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    out = mod.derive(dst=str(tmp_path / "6-31g.gbs"))
+    assert open(out).read() == open(_gbs("6-31g")).read()
+    assert "\nD " not in open(out).read()
+
+
+def test_c_abi_exports_every_declared_symbol():
+    from mmd import _native
+    header = open(os.path.join(ROOT, "include", "mmd.h")).read()
+    declared = sorted(set(re.findall(r"\b(mmd_[a-z_0-9]+)\s*\(", header)))
+    assert declared, "no declarations parsed"
+    lib = ctypes.CDLL(_native.LIB_PATH)
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert sorted(_native.EXPORTED) == declared
+    assert lib.mmd_version() == 1
+
+
+def test_context_without_device_raises():
+    from mmd import _native
+    if _native.device_count() > 0:
+        pytest.skip("a HIP device is present")
+    with pytest.raises(RuntimeError):
+        _native.Context()
+    assert "device" in _native.lib.mmd_last_error().decode()
+
+
+def test_screen_dict_matches_reference_layout():
+    from mmd.molecule import ScreenDict
+    rng = np.random.default_rng(0)
+    A = rng.random((5, 5))
+    d = ScreenDict(A + A.T)
+    assert isinstance(d, dict) and len(d) == 15
+    for p in range(5):
+        for q in range(p + 1):
+            assert d[p * (p + 1) // 2 + q] == (A + A.T)[p, q]

@@ -131,10 +131,22 @@ int mmd_jk_tensor(mmd_ctx *ctx, const double *T, int t_on_device, const double *
  * Boys function F_m(T) (boys, cython/util.pxi:53-55; test hook _boys,
  * cython/onee.pyx:195-196), evaluated on the device for n arguments:
  *   table = 0 : general real order m >= 0 (series / continued fraction)
- *   table = 1 : the tabulated kernel path (integer m <= 14)
+ *   table = 1 : the tabulated kernel path (integer m <= 8)
  * Host pointers.
  */
 int mmd_boys(mmd_ctx *ctx, int n, const double *m, const double *T, int table, double *out);
+
+/*
+ * Per-class statistics of the symmetrised Fock build.  With profiling on,
+ * every mmd_fock_delta(raw = 0) records, per shell-quartet class
+ * (la lb | lc ld): surviving shell quartets, surviving primitive shell
+ * quartets and the class kernel's device time (HIP events on the context's
+ * stream).  No counterpart in the reference (instrumentation).
+ * mmd_fock_stats fills up to 21 entries (lclass: 4 per class) and returns
+ * the number of classes.
+ */
+int mmd_set_profiling(mmd_ctx *ctx, int on);
+int mmd_fock_stats(mmd_ctx *ctx, int32_t *lclass, int64_t *nquart, int64_t *nprim, double *ms);
 
 /* Device time (ms) of the last mmd_fock_delta call, from HIP events. */
 int mmd_last_fock_ms(mmd_ctx *ctx, double *ms);

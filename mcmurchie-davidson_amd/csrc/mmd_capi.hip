@@ -118,6 +118,14 @@ struct mmd_ctx {
 
     // density / Fock scratch
     DevBuf<double> d_D, d_Dim, d_absD, d_Dsh, d_G, d_Gout;
+
+    // per-class statistics of the last symmetrised Fock build
+    bool profiling = false;
+    hipEvent_t evc[2 * NQCLASS] = {};
+    DevBuf<unsigned long long> d_nprim;
+    int64_t st_nquart[NQCLASS] = {0};
+    int64_t st_nprim[NQCLASS] = {0};
+    double st_ms[NQCLASS] = {0};
 };
 
 namespace {
@@ -331,12 +339,36 @@ int screen_all(mmd_ctx* c, double tol, int keep_all)
 
 int run_fock_classes(mmd_ctx* c, ClassArgs A)
 {
+    if (c->profiling) {
+        HIP_TRY(c->d_nprim.reserve(NQCLASS));
+        HIP_TRY(launch_fill_int(2 * NQCLASS, reinterpret_cast<int*>(c->d_nprim.p), 0, c->st));
+        for (int qc = 0; qc < NQCLASS; ++qc)
+            if (c->qc_total[qc])
+                HIP_TRY(launch_count_prims(c->d_surv.p + c->qc_off[qc], c->d_count.p + qc, c->d_pairs.p,
+                                           c->d_nprim.p + qc, c->st));
+    }
     for (int qc = 0; qc < NQCLASS; ++qc) {
         if (c->qc_total[qc] == 0) continue;
         A.quart = c->d_surv.p + c->qc_off[qc];
         A.count = c->d_count.p + qc;
         const int64_t items = ((c->qc_total[qc] + 63) / 64) * 64 * fock_class_nchunk(qc);
+        if (c->profiling) HIP_TRY(hipEventRecord(c->evc[2 * qc], c->st));
         HIP_TRY(launch_fock_class(qc, A, grid_for(items), c->st));
+        if (c->profiling) HIP_TRY(hipEventRecord(c->evc[2 * qc + 1], c->st));
+    }
+    if (c->profiling) {
+        int h[NQCLASS];
+        unsigned long long hp[NQCLASS];
+        HIP_TRY(hipMemcpyAsync(h, c->d_count.p, sizeof(int) * NQCLASS, hipMemcpyDeviceToHost, c->st));
+        HIP_TRY(hipMemcpyAsync(hp, c->d_nprim.p, sizeof(hp), hipMemcpyDeviceToHost, c->st));
+        HIP_TRY(hipStreamSynchronize(c->st));
+        for (int qc = 0; qc < NQCLASS; ++qc) {
+            c->st_nquart[qc] = c->qc_total[qc] ? h[qc] : 0;
+            c->st_nprim[qc] = c->qc_total[qc] ? (int64_t)hp[qc] : 0;
+            float ms = 0.f;
+            if (c->qc_total[qc]) HIP_TRY(hipEventElapsedTime(&ms, c->evc[2 * qc], c->evc[2 * qc + 1]));
+            c->st_ms[qc] = ms;
+        }
     }
     return MMD_OK;
 }
@@ -430,6 +462,8 @@ void mmd_ctx_destroy(mmd_ctx* ctx)
     if (ctx->st) (void)hipStreamSynchronize(ctx->st);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    for (int i = 0; i < 2 * NQCLASS; ++i)
+        if (ctx->evc[i]) (void)hipEventDestroy(ctx->evc[i]);
     hipStream_t st = ctx->st;
     delete ctx;  // frees device buffers
     if (st) (void)hipStreamDestroy(st);
@@ -730,6 +764,28 @@ int mmd_boys(mmd_ctx* ctx, int n, const double* m, const double* T, int table, d
     HIP_TRY(hipMemcpyAsync(out, d.p + 2 * n, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->st));
     HIP_TRY(hipStreamSynchronize(ctx->st));
     return MMD_OK;
+}
+
+int mmd_set_profiling(mmd_ctx* ctx, int on)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (on && !ctx->evc[0])
+        for (int i = 0; i < 2 * NQCLASS; ++i) HIP_TRY(hipEventCreate(&ctx->evc[i]));
+    ctx->profiling = on != 0;
+    return MMD_OK;
+}
+
+int mmd_fock_stats(mmd_ctx* ctx, int32_t* lclass, int64_t* nquart, int64_t* nprim, double* ms)
+{
+    if (!ctx) return fail(MMD_ERR_ARG, "null context");
+    for (int qc = 0; qc < NQCLASS; ++qc) {
+        if (lclass) qclass_decode(qc, &lclass[4 * qc], &lclass[4 * qc + 1], &lclass[4 * qc + 2], &lclass[4 * qc + 3]);
+        if (nquart) nquart[qc] = ctx->st_nquart[qc];
+        if (nprim) nprim[qc] = ctx->st_nprim[qc];
+        if (ms) ms[qc] = ctx->st_ms[qc];
+    }
+    return NQCLASS;
 }
 
 int mmd_last_fock_ms(mmd_ctx* ctx, double* ms)

@@ -50,6 +50,8 @@ hipError_t launch_screen_class(int boff, int nb, int koff, int nk, int tri, cons
 hipError_t launch_abs_density(int n, const double* Dre, const double* Dim, double* absD, hipStream_t st);
 hipError_t launch_shell_block_max(int nshell, const int* sh_bf0, const int* sh_nbf, int N, const double* absD,
                                   double* Dsh, hipStream_t st);
+hipError_t launch_count_prims(const int2* surv, const int* count, const ShellPair* pairs, unsigned long long* out,
+                              hipStream_t st);
 hipError_t launch_fill(size_t n, double* x, double v, hipStream_t st);
 hipError_t launch_fill_int(int n, int* x, int v, hipStream_t st);
 hipError_t launch_symmetrize(int N, const double* G, double* out, hipStream_t st);

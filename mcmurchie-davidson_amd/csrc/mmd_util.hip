@@ -349,6 +349,27 @@ hipError_t launch_screen_class(int boff, int nb, int koff, int nk, int tri, cons
     return hipGetLastError();
 }
 
+// Primitive shell quartets of a class's survivor list: sum of npp(bra) npp(ket).
+__global__ void count_prims_kernel(const int2* __restrict__ surv, const int* __restrict__ count,
+                                   const ShellPair* __restrict__ pairs, unsigned long long* __restrict__ out)
+{
+    const int n = *count;
+    unsigned long long s = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int2 q = surv[i];
+        s += (unsigned long long)pairs[q.x].npp * (unsigned long long)pairs[q.y].npp;
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(out, s);
+}
+
+hipError_t launch_count_prims(const int2* surv, const int* count, const ShellPair* pairs, unsigned long long* out,
+                              hipStream_t st)
+{
+    hipLaunchKernelGGL(count_prims_kernel, dim3(256), dim3(256), 0, st, surv, count, pairs, out);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Small elementwise kernels
 // ---------------------------------------------------------------------------

@@ -64,6 +64,10 @@ lib.mmd_jk_tensor.restype = _c_int
 lib.mmd_jk_tensor.argtypes = [_ctx_p, _vp, _c_int, _vp, _vp, _vp, _vp, _vp, _vp]
 lib.mmd_boys.restype = _c_int
 lib.mmd_boys.argtypes = [_ctx_p, _c_int, _vp, _vp, _c_int, _vp]
+lib.mmd_set_profiling.restype = _c_int
+lib.mmd_set_profiling.argtypes = [_ctx_p, _c_int]
+lib.mmd_fock_stats.restype = _c_int
+lib.mmd_fock_stats.argtypes = [_ctx_p, _vp, _vp, _vp, _vp]
 lib.mmd_last_fock_ms.restype = _c_int
 lib.mmd_last_fock_ms.argtypes = [_ctx_p, ctypes.POINTER(_c_double)]
 
@@ -71,7 +75,7 @@ EXPORTED = [
     "mmd_last_error", "mmd_version", "mmd_device_count", "mmd_ctx_create", "mmd_ctx_destroy",
     "mmd_set_basis", "mmd_set_atoms", "mmd_one_electron", "mmd_schwarz", "mmd_set_schwarz",
     "mmd_eri_tensor", "mmd_eri_list", "mmd_fock_delta", "mmd_jk_tensor", "mmd_boys",
-    "mmd_last_fock_ms",
+    "mmd_last_fock_ms", "mmd_set_profiling", "mmd_fock_stats",
 ]
 
 
@@ -225,6 +229,21 @@ class Context:
         out = np.empty(m.shape)
         check(lib.mmd_boys(self._h, m.size, _ptr(m), _ptr(T), 1 if table else 0, _ptr(out)))
         return out
+
+    def set_profiling(self, on=True):
+        check(lib.mmd_set_profiling(self._h, 1 if on else 0))
+
+    def fock_stats(self):
+        """Per-class (l_a, l_b, l_c, l_d), surviving shell quartets, surviving
+        primitive quartets and kernel ms of the last symmetrised build."""
+        lc = np.zeros(84, dtype=np.int32)
+        nq = np.zeros(21, dtype=np.int64)
+        npr = np.zeros(21, dtype=np.int64)
+        ms = np.zeros(21)
+        n = lib.mmd_fock_stats(self._h, _ptr(lc), _ptr(nq), _ptr(npr), _ptr(ms))
+        if n < 0:
+            check(n)
+        return lc.reshape(21, 4)[:n], nq[:n], npr[:n], ms[:n]
 
     def last_fock_ms(self):
         v = _c_double(0.0)

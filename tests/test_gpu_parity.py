@@ -271,7 +271,7 @@ def test_rhf_reference_known_answers(native):
     mol.RHF(doPrint=False, direct=True)
     assert abs(mol.energy.real - (-39.726850324347)) < 2e-8
     mol = Molecule(rescale(WATER, BOHR_1986), "dz")
-    mol.RHF(doPrint=False)
+    mol.RHF(doPrint=False, conver=1e-12)
     assert abs(mol.energy.real - (-75.977878975377)) < 1e-9
 
 

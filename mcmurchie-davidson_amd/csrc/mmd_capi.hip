@@ -353,7 +353,12 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A)
         A.count = c->d_count.p + qc;
         const int64_t items = ((c->qc_total[qc] + 63) / 64) * 64 * fock_class_nchunk(qc);
         if (c->profiling) HIP_TRY(hipEventRecord(c->evc[2 * qc], c->st));
-        HIP_TRY(launch_fock_class(qc, A, grid_for(items), c->st));
+        int la, lb, lc, ld;
+        qclass_decode(qc, &la, &lb, &lc, &ld);
+        if (fock_class_is_wave(la, lb, lc, ld))
+            HIP_TRY(launch_fock_wave_class(la, lb, lc, ld, A, c->qc_total[qc], c->st));
+        else
+            HIP_TRY(launch_fock_class(qc, A, grid_for(items), c->st));
         if (c->profiling) HIP_TRY(hipEventRecord(c->evc[2 * qc + 1], c->st));
     }
     if (c->profiling) {

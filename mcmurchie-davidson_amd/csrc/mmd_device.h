@@ -5,6 +5,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
+#include <utility>
 
 namespace mmd {
 
@@ -48,6 +50,36 @@ constexpr int MAX_L = 2;            // highest shell angular momentum supported
 constexpr int MAX_LTOT = 4 * MAX_L; // highest total L of a quartet
 constexpr int NPAIRCLASS = 6;
 constexpr int NQCLASS = 21;         // quartet classes with pair_class(bra) >= pair_class(ket)
+
+// ---------------------------------------------------------------------------
+// Compile-time loops
+// ---------------------------------------------------------------------------
+template <class F, int... I>
+__device__ __forceinline__ void sfor_impl(F&& f, std::integer_sequence<int, I...>)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void sfor(F&& f)
+{
+    if constexpr (N > 0) sfor_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+struct Herm3 {
+    int t, u, v;
+};
+// Inverse of hidx: (t,u,v) of flat Hermite index k.
+__host__ __device__ constexpr Herm3 hdecode(int k)
+{
+    int n = 0;
+    while (nherm(n) <= k) ++n;
+    int r = k - (n > 0 ? nherm(n - 1) : 0);
+    int s = 0;
+    while ((s + 1) * (s + 2) / 2 <= r) ++s;
+    const int v = r - s * (s + 1) / 2;
+    return Herm3{n - s, s - v, v};
+}
+
 
 // ---------------------------------------------------------------------------
 // Boys function table
@@ -126,32 +158,30 @@ template <int L>
 __device__ inline void hermite_R(double X, double Y, double Z, const double* F, double* R)
 {
     R[0] = F[L];
-#pragma unroll
-    for (int n = L - 1; n >= 0; --n) {
-#pragma unroll
-        for (int d = L - n; d >= 1; --d) {
-#pragma unroll
-            for (int t = d; t >= 0; --t) {
-#pragma unroll
-                for (int u = d - t; u >= 0; --u) {
-                    const int v = d - t - u;
-                    double val;
-                    if (t > 0) {
-                        val = X * R[hidx(t - 1, u, v)];
-                        if (t > 1) val += (double)(t - 1) * R[hidx(t - 2, u, v)];
-                    } else if (u > 0) {
-                        val = Y * R[hidx(t, u - 1, v)];
-                        if (u > 1) val += (double)(u - 1) * R[hidx(t, u - 2, v)];
-                    } else {
-                        val = Z * R[hidx(t, u, v - 1)];
-                        if (v > 1) val += (double)(v - 1) * R[hidx(t, u, v - 2)];
-                    }
-                    R[hidx(t, u, v)] = val;
+    sfor<L>([&](auto in) {
+        constexpr int n = L - 1 - decltype(in)::value;
+        sfor<L - n>([&](auto id) {
+            constexpr int d = L - n - decltype(id)::value;       // degree L-n down to 1
+            sfor<nherm(d) - nherm(d - 1)>([&](auto ik) {
+                constexpr int k = nherm(d - 1) + decltype(ik)::value;
+                constexpr Herm3 h = hdecode(k);
+                constexpr int t = h.t, u = h.u, v = h.v;
+                double val;
+                if constexpr (t > 0) {
+                    val = X * R[hidx(t - 1, u, v)];
+                    if constexpr (t > 1) val = fma((double)(t - 1), R[hidx(t - 2, u, v)], val);
+                } else if constexpr (u > 0) {
+                    val = Y * R[hidx(t, u - 1, v)];
+                    if constexpr (u > 1) val = fma((double)(u - 1), R[hidx(t, u - 2, v)], val);
+                } else {
+                    val = Z * R[hidx(t, u, v - 1)];
+                    if constexpr (v > 1) val = fma((double)(v - 1), R[hidx(t, u, v - 2)], val);
                 }
-            }
-        }
+                R[k] = val;
+            });
+        });
         R[0] = F[n];
-    }
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -167,44 +197,45 @@ template <int LI, int LJ, int SIGN>
 __device__ inline void hermite_E(double inv2p, double PA, double PB, double* E)
 {
     constexpr int NT = LI + LJ + 1;
-#define EIDX(i, j, t) (((i) * (LJ + 1) + (j)) * NT + (t))
-#pragma unroll
-    for (int k = 0; k < (LI + 1) * (LJ + 1) * NT; ++k) E[k] = 0.0;
-    E[EIDX(0, 0, 0)] = 1.0;
-#pragma unroll
-    for (int i = 0; i < LI; ++i) {
-#pragma unroll
-        for (int t = 0; t <= i + 1; ++t) {
+    constexpr auto EI = [](int i, int j, int t) { return (i * (LJ + 1) + j) * NT + t; };
+    sfor<(LI + 1) * (LJ + 1) * NT>([&](auto k) { E[k] = 0.0; });
+    E[0] = 1.0;
+    sfor<LI>([&](auto ii) {
+        constexpr int i = decltype(ii)::value;
+        sfor<i + 2>([&](auto it) {
+            constexpr int t = decltype(it)::value;
             double v = 0.0;
-            if (t > 0) v += inv2p * E[EIDX(i, 0, t - 1)];
-            if (t <= i) v += PA * E[EIDX(i, 0, t)];
-            if (t + 1 <= i) v += (double)(t + 1) * E[EIDX(i, 0, t + 1)];
-            E[EIDX(i + 1, 0, t)] = v;
-        }
-    }
-#pragma unroll
-    for (int i = 0; i <= LI; ++i) {
-#pragma unroll
-        for (int j = 0; j < LJ; ++j) {
-#pragma unroll
-            for (int t = 0; t <= i + j + 1; ++t) {
+            if constexpr (t > 0) v = fma(inv2p, E[EI(i, 0, t - 1)], v);
+            if constexpr (t <= i) v = fma(PA, E[EI(i, 0, t)], v);
+            if constexpr (t + 1 <= i) v = fma((double)(t + 1), E[EI(i, 0, t + 1)], v);
+            E[EI(i + 1, 0, t)] = v;
+        });
+    });
+    sfor<LI + 1>([&](auto ii) {
+        constexpr int i = decltype(ii)::value;
+        sfor<LJ>([&](auto ij) {
+            constexpr int j = decltype(ij)::value;
+            sfor<i + j + 2>([&](auto it) {
+                constexpr int t = decltype(it)::value;
                 double v = 0.0;
-                if (t > 0) v += inv2p * E[EIDX(i, j, t - 1)];
-                if (t <= i + j) v += PB * E[EIDX(i, j, t)];
-                if (t + 1 <= i + j) v += (double)(t + 1) * E[EIDX(i, j, t + 1)];
-                E[EIDX(i, j + 1, t)] = v;
-            }
-        }
+                if constexpr (t > 0) v = fma(inv2p, E[EI(i, j, t - 1)], v);
+                if constexpr (t <= i + j) v = fma(PB, E[EI(i, j, t)], v);
+                if constexpr (t + 1 <= i + j) v = fma((double)(t + 1), E[EI(i, j, t + 1)], v);
+                E[EI(i, j + 1, t)] = v;
+            });
+        });
+    });
+    if constexpr (SIGN < 0) {
+        sfor<LI + 1>([&](auto ii) {
+            sfor<LJ + 1>([&](auto ij) {
+                constexpr int i = decltype(ii)::value, j = decltype(ij)::value;
+                sfor<(i + j + 1) / 2>([&](auto it) {
+                    constexpr int t = 2 * decltype(it)::value + 1;
+                    E[EI(i, j, t)] = -E[EI(i, j, t)];
+                });
+            });
+        });
     }
-    if (SIGN < 0) {
-#pragma unroll
-        for (int i = 0; i <= LI; ++i)
-#pragma unroll
-            for (int j = 0; j <= LJ; ++j)
-#pragma unroll
-                for (int t = 1; t <= i + j; t += 2) E[EIDX(i, j, t)] = -E[EIDX(i, j, t)];
-    }
-#undef EIDX
 }
 
 // ---------------------------------------------------------------------------

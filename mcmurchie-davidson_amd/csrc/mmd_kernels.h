@@ -23,6 +23,10 @@ inline __host__ __device__ void qclass_decode(int qc, int* la, int* lb, int* lc,
 }
 
 hipError_t launch_fock_class(int qc, const ClassArgs& A, int grid, hipStream_t st);
+hipError_t launch_fock_wave_class(int la, int lb, int lc, int ld, const ClassArgs& A, int64_t max_quartets,
+                                  hipStream_t st);
+// Classes served by the wavefront-per-quartet kernel instead of a unrolled class kernel.
+inline bool fock_class_is_wave(int la, int lb, int lc, int ld) { return la == 2 && lb == 2 && lc == 2 && ld == 2; }
 int fock_class_nchunk(int qc);
 
 // A single Cartesian basis function for the generic (per basis-function

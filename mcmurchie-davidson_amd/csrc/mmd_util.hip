@@ -371,6 +371,135 @@ hipError_t launch_count_prims(const int2* surv, const int* count, const ShellPai
 }
 
 // ---------------------------------------------------------------------------
+// Fock build for a shell-quartet class with runtime angular momenta: one
+// wavefront per shell quartet.  Per primitive quartet, lane 0 evaluates the
+// Boys function and the R_tuv recursion into LDS and lanes 0-5 the 1-D
+// Hermite tables; then every lane contracts its share of the component
+// quartets (ab|cd) from LDS, accumulating over primitives in registers.
+// Digestion as the class kernels (shell degeneracy, basis-function Schwarz
+// x |dP| test, Coulomb and exchange updates).  Used for the high-L classes
+// whose fully unrolled kernels are not worth their code size.
+// ---------------------------------------------------------------------------
+constexpr int GW_WAVES = 4;                         // waves per block
+constexpr int GW_MAXC = (6 * 6 * 6 * 6 + 63) / 64;  // components per lane (d-d|d-d)
+
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ void __launch_bounds__(64 * GW_WAVES) fock_wave_class_kernel(ClassArgs A, int la, int lb, int lc, int ld)
+{
+    __shared__ double Rs[GW_WAVES][nherm(MAX_LTOT)];
+    __shared__ double Es[GW_WAVES][6][(MAX_L + 1) * (MAX_L + 1) * (2 * MAX_L + 1)];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nq = *A.count;
+    const int na = ncart(la), nb = ncart(lb), nc = ncart(lc), nd = ncart(ld);
+    const int nab = na * nb, ncd = nc * nd, ntot = nab * ncd;
+    const int L = la + lb + lc + ld;
+    const int N = A.N;
+    double* R = Rs[wv];
+    for (int64_t q = (int64_t)blockIdx.x * GW_WAVES + wv; q < nq; q += (int64_t)gridDim.x * GW_WAVES) {
+        const int2 qd = A.quart[q];
+        const ShellPair bra = A.pairs[qd.x];
+        const ShellPair ket = A.pairs[qd.y];
+        double acc[GW_MAXC];
+        for (int i = 0; i < GW_MAXC; ++i) acc[i] = 0.0;
+        for (int ip = 0; ip < bra.npp; ++ip) {
+            const PrimPair* P = A.pp + bra.pp0 + ip;
+            for (int jq = 0; jq < ket.npp; ++jq) {
+                const PrimPair* Q = A.pp + ket.pp0 + jq;
+                if (lane == 0) {
+                    const double pq = P->p * Q->p, ppq = P->p + Q->p;
+                    const double alpha = pq / ppq;
+                    const double X = P->Px - Q->Px, Y = P->Py - Q->Py, Z = P->Pz - Q->Pz;
+                    double F[MAX_LTOT + 1];
+                    boys_runtime(L, alpha * (X * X + Y * Y + Z * Z), A.boys, F);
+                    double s = 34.98683665524972497 / (pq * sqrt(ppq)) * P->K * Q->K;
+                    for (int m = 0; m <= L; ++m) { F[m] *= s; s *= -2.0 * alpha; }
+                    hermite_R_rt(L, X, Y, Z, F, R);
+                } else if (lane <= 6) {
+                    const int x = (lane - 1) % 3;
+                    const bool isbra = lane <= 3;
+                    const PrimPair* S = isbra ? P : Q;
+                    const double PA = x == 0 ? S->PAx : (x == 1 ? S->PAy : S->PAz);
+                    const double PB = x == 0 ? S->PBx : (x == 1 ? S->PBy : S->PBz);
+                    etab_rt(isbra ? la : lc, isbra ? lb : ld, S->inv2p, PA, PB, Es[wv][lane - 1]);
+                }
+                wave_sync();
+                const int ntb = la + lb + 1, ntk = lc + ld + 1;
+                for (int i = 0; i < GW_MAXC; ++i) {
+                    const int k = lane + 64 * i;
+                    if (k >= ntot) break;
+                    const int ab = k / ncd, cd = k % ncd;
+                    const int a = ab / nb, b = ab % nb, c = cd / nd, d = cd % nd;
+                    const int ax = comp_lx(la, a), ay = comp_ly(la, a), az = comp_lz(la, a);
+                    const int bx = comp_lx(lb, b), by = comp_ly(lb, b), bz = comp_lz(lb, b);
+                    const int cx = comp_lx(lc, c), cy = comp_ly(lc, c), cz = comp_lz(lc, c);
+                    const int dx = comp_lx(ld, d), dy = comp_ly(ld, d), dz = comp_lz(ld, d);
+                    const double* ebx = Es[wv][0] + (ax * (lb + 1) + bx) * ntb;
+                    const double* eby = Es[wv][1] + (ay * (lb + 1) + by) * ntb;
+                    const double* ebz = Es[wv][2] + (az * (lb + 1) + bz) * ntb;
+                    const double* ekx = Es[wv][3] + (cx * (ld + 1) + dx) * ntk;
+                    const double* eky = Es[wv][4] + (cy * (ld + 1) + dy) * ntk;
+                    const double* ekz = Es[wv][5] + (cz * (ld + 1) + dz) * ntk;
+                    double v = 0.0;
+                    for (int t = 0; t <= ax + bx; ++t)
+                        for (int u = 0; u <= ay + by; ++u)
+                            for (int w = 0; w <= az + bz; ++w) {
+                                double h = 0.0;
+                                for (int tau = 0; tau <= cx + dx; ++tau)
+                                    for (int nu = 0; nu <= cy + dy; ++nu) {
+                                        const double e2 = ((tau + nu) & 1 ? -1.0 : 1.0) * ekx[tau] * eky[nu];
+                                        for (int phi = 0; phi <= cz + dz; ++phi)
+                                            h += (phi & 1 ? -e2 : e2) * ekz[phi] * R[hidx(t + tau, u + nu, w + phi)];
+                                    }
+                                v += ebx[t] * eby[u] * ebz[w] * h;
+                            }
+                    acc[i] += v;
+                }
+                wave_sync();
+            }
+        }
+        const int bfa = bra.bfa, bfb = bra.bfb, bfc = ket.bfa, bfd = ket.bfb;
+        const double deg = (bra.same ? 1.0 : 2.0) * (ket.same ? 1.0 : 2.0) * (qd.x == qd.y ? 1.0 : 2.0);
+        for (int i = 0; i < GW_MAXC; ++i) {
+            const int k = lane + 64 * i;
+            if (k >= ntot) break;
+            const int ab = k / ncd, cd = k % ncd;
+            const int p = bfa + ab / nb, qq = bfb + ab % nb, r = bfc + cd / nd, s = bfd + cd % nd;
+            const double* aD = A.absD;
+            const double bound = A.sqS[(size_t)p * N + qq] * A.sqS[(size_t)r * N + s]
+                * fmax(fmax(fmax(4.0 * aD[(size_t)p * N + qq], 4.0 * aD[(size_t)r * N + s]),
+                            fmax(aD[(size_t)p * N + r], aD[(size_t)p * N + s])),
+                       fmax(aD[(size_t)qq * N + r], aD[(size_t)qq * N + s]));
+            if (bound < A.tol) continue;
+            const double v = acc[i] * deg * A.gcomp[p] * A.gcomp[qq] * A.gcomp[r] * A.gcomp[s];
+            const double* D = A.D;
+            double* G = A.G;
+            atomicAdd(&G[(size_t)p * N + qq], v * D[(size_t)r * N + s]);
+            atomicAdd(&G[(size_t)r * N + s], v * D[(size_t)p * N + qq]);
+            atomicAdd(&G[(size_t)qq * N + s], -0.25 * v * D[(size_t)p * N + r]);
+            atomicAdd(&G[(size_t)qq * N + r], -0.25 * v * D[(size_t)p * N + s]);
+            atomicAdd(&G[(size_t)p * N + r], -0.25 * v * D[(size_t)qq * N + s]);
+            atomicAdd(&G[(size_t)p * N + s], -0.25 * v * D[(size_t)qq * N + r]);
+        }
+    }
+}
+
+hipError_t launch_fock_wave_class(int la, int lb, int lc, int ld, const ClassArgs& A, int64_t max_quartets,
+                                  hipStream_t st)
+{
+    int64_t grid = (max_quartets + GW_WAVES - 1) / GW_WAVES;
+    if (grid < 1) grid = 1;
+    if (grid > 4096) grid = 4096;
+    hipLaunchKernelGGL(fock_wave_class_kernel, dim3((unsigned)grid), dim3(64 * GW_WAVES), 0, st, A, la, lb, lc, ld);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Small elementwise kernels
 // ---------------------------------------------------------------------------
 __global__ void abs_density_kernel(int n, const double* __restrict__ Dre, const double* __restrict__ Dim,

@@ -9,7 +9,6 @@ import os
 import sys
 
 import numpy as np
-from scipy.linalg import fractional_matrix_power as _mat_pow
 
 from mmd import _native
 from mmd.integrals.twoe import Basis
@@ -210,8 +209,11 @@ class Molecule(SCF):
         for a, b in itertools.combinations(self.atoms, 2):
             self.nuc_energy += a.charge * b.charge / np.linalg.norm(a.origin - b.origin)
         self.Core = self.T + self.V
-        self.X = _mat_pow(self.S, -0.5)
-        self.U = _mat_pow(self.S, 0.5)
+        # S^(-1/2) and S^(1/2), the principal powers of the symmetric positive
+        # definite overlap (fractional_matrix_power in mmd/molecule.py:285-286)
+        s, V = np.linalg.eigh(self.S)
+        self.X = np.dot(V * s ** -0.5, V.T)
+        self.U = np.dot(V * s ** 0.5, V.T)
         assert self.S.shape == (N, N)
 
     def two_electron_integrals(self):

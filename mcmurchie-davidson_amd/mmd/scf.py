@@ -9,9 +9,17 @@ the stored tensor on the GPU (mmd/scf.py:97-98).
 """
 import numpy as np
 import scipy.linalg
-from numpy.linalg import multi_dot as dot
 
 from mmd.integrals.fock import formPT
+
+
+def _re(A):
+    """A itself, or its real part when the imaginary part is identically zero
+    (the closed-shell SCF is real; arithmetic then runs in real FP64)."""
+    A = np.asarray(A)
+    if np.iscomplexobj(A) and not np.any(A.imag):
+        return np.ascontiguousarray(A.real)
+    return A
 
 
 class SCF(object):
@@ -46,16 +54,17 @@ class SCF(object):
                 self.buildFock()
                 self.P_old = self.P
                 if DIIS:
-                    F_diis = self.updateDIIS(self.F, self.P)
+                    F_diis = _re(self.updateDIIS(self.F, self.P))
                     self.FO = np.dot(self.X.T, np.dot(F_diis, self.X))
             if not DIIS or step == 0:
                 self.orthoFock()
 
-            E, self.CO = scipy.linalg.eigh(self.FO)
+            E, self.CO = scipy.linalg.eigh(_re(self.FO))
             C = np.dot(self.X, self.CO)
             self.C = C
             self.MO = E
-            self.P = np.dot(C[:, :self.nocc], np.conjugate(C[:, :self.nocc]).T)
+            Cocc = C[:, :self.nocc]
+            self.P = np.dot(Cocc, np.conjugate(Cocc).T).astype('complex')
             self.computeEnergy()
 
             if step > 0:
@@ -121,7 +130,7 @@ class SCF(object):
             self.F = self.Core.astype('complex') + self.G
 
     def orthoFock(self):
-        self.FO = np.dot(self.X.T, np.dot(self.F, self.X))
+        self.FO = np.dot(self.X.T, np.dot(_re(self.F), self.X))
 
     def unOrthoFock(self):
         self.F = np.dot(self.U.T, np.dot(self.FO, self.U))
@@ -160,10 +169,11 @@ class SCF(object):
 
     def updateDIIS(self, F, P):
         """Pulay DIIS on the orthogonalised FPS - SPF, subspace of 8
-        (mmd/scf.py:146-180)."""
-        FPS = dot([F, P, self.S])
+        (mmd/scf.py:146-180).  B_ij = tr(e_i^H e_j) is evaluated as the
+        elementwise inner product (same value, O(N^2))."""
+        FPS = np.dot(_re(F), np.dot(_re(P), self.S))
         SPF = self.adj(FPS)
-        error = dot([self.X, FPS - SPF, self.X])
+        error = np.dot(self.X, np.dot(FPS - SPF, self.X))
         self.fockSet.append(self.F)
         self.errorSet.append(error)
         numFock = len(self.fockSet)
@@ -176,7 +186,7 @@ class SCF(object):
         B[-1, -1] = 0.0
         for i in range(numFock):
             for j in range(i + 1):
-                B[i, j] = B[j, i] = np.real(np.trace(np.dot(self.adj(self.errorSet[i]), self.errorSet[j])))
+                B[i, j] = B[j, i] = np.real(np.vdot(self.errorSet[i], self.errorSet[j]))
         residual = np.zeros(numFock + 1)
         residual[-1] = -1.0
         weights = np.linalg.solve(B, residual)

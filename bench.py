@@ -61,7 +61,8 @@ def run_scf(geom, basis, profile=False):
 
 
 def roofline_from_stats(stats):
-    """Per class: (model flops, kernel ms, launches) summed over builds."""
+    """Per class: (model flops, kernel ms, launches, shell quartets, primitive
+    quartets) summed over builds."""
     per = {}
     for lc, nq, npr, ms in stats:
         for c in range(len(nq)):
@@ -70,8 +71,8 @@ def roofline_from_stats(stats):
                 continue
             la, lb, lcc, ld = key
             fl = npr[c] * class_flops(*key) + 12.0 * nq[c] * ncart(la) * ncart(lb) * ncart(lcc) * ncart(ld)
-            f, t, n = per.get(key, (0.0, 0.0, 0))
-            per[key] = (f + fl, t + ms[c], n + 1)
+            f, t, n, sq, sp = per.get(key, (0.0, 0.0, 0, 0, 0))
+            per[key] = (f + fl, t + ms[c], n + 1, sq + int(nq[c]), sp + int(npr[c]))
     return per
 
 
@@ -169,7 +170,7 @@ def main():
         tot_f = sum(v[0] for v in per.values())
         tot_t = sum(v[1] for v in per.values())
         dom = max(per, key=lambda k: per[k][1])
-        f, t, n = per[dom]
+        f, t, n = per[dom][:3]
         achieved = f / (t * 1e-3) / 1e12
         out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
@@ -177,8 +178,11 @@ def main():
                            "kernel_avg_ms": t / n, "launches": n,
                            "all_classes_achieved": tot_f / (tot_t * 1e-3) / 1e12,
                            "all_classes_ms_per_build": tot_t / max(1, len(pm._profile_stats))}
-        out["class_breakdown"] = {"(%d%d|%d%d)" % k: {"ms": v[1], "tflops": v[0] / (v[1] * 1e-3) / 1e12 if v[1] else 0}
-                                  for k, v in sorted(per.items(), key=lambda kv: -kv[1][1])}
+        nb = max(1, len(pm._profile_stats))
+        out["class_breakdown"] = {
+            "(%d%d|%d%d)" % k: {"ms_per_build": v[1] / nb, "tflops": v[0] / (v[1] * 1e-3) / 1e12 if v[1] else 0,
+                                "shell_quartets_per_build": v[3] // nb, "prim_quartets_per_build": v[4] // nb}
+            for k, v in sorted(per.items(), key=lambda kv: -kv[1][1])}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(geom, basis, len(m0.fock_ms), args.cpu_seconds)
     print(json.dumps(out))

@@ -97,6 +97,7 @@ struct mmd_ctx {
 
     // shell pairs (grouped by pair class, sorted inside each class)
     bool pairs_ok = false;
+    double prim_qmax = 0.0;
     std::vector<ShellPair> pairs;
     int pc_off[NPAIRCLASS + 1] = {0};
     DevBuf<ShellPair> d_pairs;
@@ -122,7 +123,9 @@ struct mmd_ctx {
     // per-class statistics of the last symmetrised Fock build
     bool profiling = false;
     hipEvent_t evc[2 * NQCLASS] = {};
-    DevBuf<unsigned long long> d_nprim;
+    DevBuf<unsigned long long> d_nprim;   // [0,21): primitive quartets, [21,42): shell quartets
+    DevBuf<int> d_rowctr;
+    size_t max_lds = 65536;
     int64_t st_nquart[NQCLASS] = {0};
     int64_t st_nprim[NQCLASS] = {0};
     double st_ms[NQCLASS] = {0};
@@ -211,11 +214,41 @@ int upload_shells(mmd_ctx* c)
     return MMD_OK;
 }
 
+// Schwarz-type bound of a primitive pair's product function (any component):
+// sqrt([ab|ab]) of an s-type pair, 2 pi^{5/2} / (p^2 sqrt(2p)) K^2, times
+// (1 + |AB| + 1/sqrt(p))^{la+lb} 4^{la+lb} to cover the Hermite expansion of
+// higher angular momenta.
+double prim_bound(double p, double K, double AB2, int lab)
+{
+    const double s = sqrt(34.98683665524972497 / (p * p * sqrt(2.0 * p))) * fabs(K);
+    const double f = 4.0 * (1.0 + sqrt(AB2) + 1.0 / sqrt(p));
+    double m = 1.0;
+    for (int i = 0; i < lab; ++i) m *= f;
+    return s * m;
+}
+constexpr double PRIM_DROP = 1e-20;
+
 // Shell pairs I >= J in canonical orientation, with primitive-pair data.
-// `order` (optional): sort inside each pair class by (npp desc, q desc).
+// `qpair` (optional): Schwarz factors; pairs are then sorted inside each
+// pair class by (npp desc, q desc).
 int build_pairs(mmd_ctx* c, const std::vector<double>* qpair)
 {
     const int ns = (int)c->shells.size();
+    // largest primitive-pair bound, for the primitive-pair screen
+    c->prim_qmax = 0.0;
+    for (int I = 0; I < ns; ++I)
+        for (int J = 0; J <= I; ++J) {
+            const Shell& A = c->shells[I];
+            const Shell& B = c->shells[J];
+            const double AB2 = (A.x - B.x) * (A.x - B.x) + (A.y - B.y) * (A.y - B.y) + (A.z - B.z) * (A.z - B.z);
+            for (int i = 0; i < A.np; ++i)
+                for (int j = 0; j < B.np; ++j) {
+                    const double ea = c->sh_exp[A.p0 + i], eb = c->sh_exp[B.p0 + j];
+                    const double p = ea + eb;
+                    const double K = c->sh_w[A.p0 + i] * c->sh_w[B.p0 + j] * exp(-ea * eb / p * AB2);
+                    c->prim_qmax = std::max(c->prim_qmax, prim_bound(p, K, AB2, A.l + B.l));
+                }
+        }
     std::vector<std::vector<ShellPair>> byclass(NPAIRCLASS);
     std::vector<PrimPair> pp;
     int pidx = 0;
@@ -231,7 +264,6 @@ int build_pairs(mmd_ctx* c, const std::vector<double>* qpair)
             sp.bfa = A.bf0;
             sp.bfb = B.bf0;
             sp.pp0 = (int)pp.size();
-            sp.npp = A.np * B.np;
             sp.same = (a == b) ? 1 : 0;
             sp.pad = pidx;
             sp.q = qpair ? (*qpair)[pidx] : 1.0;
@@ -242,6 +274,11 @@ int build_pairs(mmd_ctx* c, const std::vector<double>* qpair)
                     PrimPair q;
                     q.p = ea + eb;
                     q.K = c->sh_w[A.p0 + i] * c->sh_w[B.p0 + j] * exp(-ea * eb / q.p * AB2);
+                    // Primitive-pair screening (Fock-build tables only): a pair
+                    // whose Schwarz-type bound, times the largest bound of any
+                    // pair, is below PRIM_DROP contributes less than that to
+                    // every primitive quartet and is left out.
+                    if (prim_bound(q.p, q.K, AB2, A.l + B.l) * c->prim_qmax < PRIM_DROP) continue;
                     q.Px = (ea * A.x + eb * B.x) / q.p;
                     q.Py = (ea * A.y + eb * B.y) / q.p;
                     q.Pz = (ea * A.z + eb * B.z) / q.p;
@@ -252,9 +289,11 @@ int build_pairs(mmd_ctx* c, const std::vector<double>* qpair)
                     q.PBy = q.Py - B.y;
                     q.PBz = q.Pz - B.z;
                     q.inv2p = 0.5 / q.p;
-                    q.pad = 0.0;
+                    q.invp = 1.0 / q.p;
                     pp.push_back(q);
                 }
+            sp.npp = (int)pp.size() - sp.pp0;
+            if (sp.npp == 0) sp.q = 0.0;   // no primitive pair survives: never screened in
             byclass[pair_class(A.l, B.l)].push_back(sp);
         }
     c->pairs.clear();
@@ -320,55 +359,71 @@ ClassArgs base_args(mmd_ctx* c)
     return A;
 }
 
-// Screen (or take all) quartets of every class into the survivor lists.
-int screen_all(mmd_ctx* c, double tol, int keep_all)
+// Launch the Fock kernels of every class.  Classes with a row kernel screen
+// inline; the wavefront-per-quartet classes (and any class whose bra rows do
+// not fit in LDS) are screened into survivor lists first.
+int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol)
 {
+    HIP_TRY(c->d_rowctr.reserve(NQCLASS));
+    HIP_TRY(c->d_nprim.reserve(2 * NQCLASS));
+    HIP_TRY(launch_fill_int(NQCLASS, c->d_rowctr.p, 0, c->st));
     HIP_TRY(launch_fill_int(NQCLASS, c->d_count.p, 0, c->st));
+    HIP_TRY(launch_fill_int(4 * NQCLASS, reinterpret_cast<int*>(c->d_nprim.p), 0, c->st));
+    bool row_kernel[NQCLASS];
     for (int qc = 0; qc < NQCLASS; ++qc) {
-        if (c->qc_total[qc] == 0) continue;
-        int pcb, pck;
-        class_of(qc, &pcb, &pck);
-        const int nb = c->pc_off[pcb + 1] - c->pc_off[pcb];
-        const int nk = c->pc_off[pck + 1] - c->pc_off[pck];
-        HIP_TRY(launch_screen_class(c->pc_off[pcb], nb, c->pc_off[pck], nk, pcb == pck ? 1 : 0, c->d_pairs.p,
-                                    c->d_Dsh.p, (int)c->shells.size(), tol, keep_all, c->d_surv.p + c->qc_off[qc],
-                                    c->d_count.p + qc, c->st));
-    }
-    return MMD_OK;
-}
-
-int run_fock_classes(mmd_ctx* c, ClassArgs A)
-{
-    if (c->profiling) {
-        HIP_TRY(c->d_nprim.reserve(NQCLASS));
-        HIP_TRY(launch_fill_int(2 * NQCLASS, reinterpret_cast<int*>(c->d_nprim.p), 0, c->st));
-        for (int qc = 0; qc < NQCLASS; ++qc)
-            if (c->qc_total[qc])
-                HIP_TRY(launch_count_prims(c->d_surv.p + c->qc_off[qc], c->d_count.p + qc, c->d_pairs.p,
-                                           c->d_nprim.p + qc, c->st));
-    }
-    for (int qc = 0; qc < NQCLASS; ++qc) {
-        if (c->qc_total[qc] == 0) continue;
-        A.quart = c->d_surv.p + c->qc_off[qc];
-        A.count = c->d_count.p + qc;
-        const int64_t items = ((c->qc_total[qc] + 63) / 64) * 64 * fock_class_nchunk(qc);
-        if (c->profiling) HIP_TRY(hipEventRecord(c->evc[2 * qc], c->st));
         int la, lb, lc, ld;
         qclass_decode(qc, &la, &lb, &lc, &ld);
-        if (fock_class_is_wave(la, lb, lc, ld))
+        row_kernel[qc] = !fock_class_is_wave(la, lb, lc, ld) && fock_class_lds(qc, c->nbf) <= c->max_lds;
+    }
+    for (int qc = 0; qc < NQCLASS; ++qc) {
+        if (c->qc_total[qc] == 0) continue;
+        int pcb, pck, la, lb, lc, ld;
+        class_of(qc, &pcb, &pck);
+        qclass_decode(qc, &la, &lb, &lc, &ld);
+        const int nb = c->pc_off[pcb + 1] - c->pc_off[pcb];
+        const int nk = c->pc_off[pck + 1] - c->pc_off[pck];
+        if (c->profiling) HIP_TRY(hipEventRecord(c->evc[2 * qc], c->st));
+        if (row_kernel[qc]) {
+            RowArgs R;
+            R.boff = c->pc_off[pcb];
+            R.nb = nb;
+            R.koff = c->pc_off[pck];
+            R.nk = nk;
+            R.tri = pcb == pck ? 1 : 0;
+            R.nshell = (int)c->shells.size();
+            R.Dsh = c->d_Dsh.p;
+            R.row_counter = c->d_rowctr.p + qc;
+            // ket segment: enough kets for the workgroup's four waves, short
+            // enough for a few thousand work units per class
+            const int nch = fock_class_nchunk(qc);
+            int seg = ((4 + nch - 1) / nch) * 64;
+            const int64_t quartets = R.tri ? (int64_t)nb * (nb + 1) / 2 : (int64_t)nb * nk;
+            while (seg > 64 && quartets / seg < 4096) seg -= 64;
+            R.seg = seg;
+            const int64_t units = (int64_t)nb * ((nk + seg - 1) / seg);
+            R.stat_nquart = c->d_nprim.p + NQCLASS + qc;
+            R.stat_nprim = c->profiling ? c->d_nprim.p + qc : nullptr;
+            HIP_TRY(launch_fock_class(qc, A, R, (int)std::min<int64_t>(units, 2048), c->st));
+        } else {
+            HIP_TRY(launch_screen_class(c->pc_off[pcb], nb, c->pc_off[pck], nk, pcb == pck ? 1 : 0, c->d_pairs.p,
+                                        c->d_Dsh.p, (int)c->shells.size(), tol, 0, c->d_surv.p + c->qc_off[qc],
+                                        c->d_count.p + qc, c->st));
+            A.quart = c->d_surv.p + c->qc_off[qc];
+            A.count = c->d_count.p + qc;
+            if (c->profiling)
+                HIP_TRY(launch_count_prims(A.quart, A.count, c->d_pairs.p, c->d_nprim.p + qc, c->st));
             HIP_TRY(launch_fock_wave_class(la, lb, lc, ld, A, c->qc_total[qc], c->st));
-        else
-            HIP_TRY(launch_fock_class(qc, A, grid_for(items), c->st));
+        }
         if (c->profiling) HIP_TRY(hipEventRecord(c->evc[2 * qc + 1], c->st));
     }
     if (c->profiling) {
         int h[NQCLASS];
-        unsigned long long hp[NQCLASS];
+        unsigned long long hp[2 * NQCLASS];
         HIP_TRY(hipMemcpyAsync(h, c->d_count.p, sizeof(int) * NQCLASS, hipMemcpyDeviceToHost, c->st));
         HIP_TRY(hipMemcpyAsync(hp, c->d_nprim.p, sizeof(hp), hipMemcpyDeviceToHost, c->st));
         HIP_TRY(hipStreamSynchronize(c->st));
         for (int qc = 0; qc < NQCLASS; ++qc) {
-            c->st_nquart[qc] = c->qc_total[qc] ? h[qc] : 0;
+            c->st_nquart[qc] = c->qc_total[qc] ? (row_kernel[qc] ? (int64_t)hp[NQCLASS + qc] : h[qc]) : 0;
             c->st_nprim[qc] = c->qc_total[qc] ? (int64_t)hp[qc] : 0;
             float ms = 0.f;
             if (c->qc_total[qc]) HIP_TRY(hipEventElapsedTime(&ms, c->evc[2 * qc], c->evc[2 * qc + 1]));
@@ -445,6 +500,10 @@ int mmd_ctx_create(mmd_ctx** out, int device_id)
     mmd_ctx* c = new (std::nothrow) mmd_ctx();
     if (!c) return fail(MMD_ERR_NOMEM, "context allocation failed");
     c->device = device_id;
+    int lds_max = 0;
+    if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device_id) == hipSuccess &&
+        lds_max > 0)
+        c->max_lds = std::min<size_t>((size_t)lds_max, 65536);
     hipError_t e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
@@ -679,8 +738,6 @@ int mmd_fock_delta(mmd_ctx* ctx, const double* dP_re, const double* dP_im, doubl
     if (!raw) {
         HIP_TRY(launch_shell_block_max((int)ctx->shells.size(), ctx->d_sh_bf0.p, ctx->d_sh_nbf.p, N, ctx->d_absD.p,
                                        ctx->d_Dsh.p, ctx->st));
-        rc = screen_all(ctx, tol, 0);
-        if (rc) return rc;
         ClassArgs A = base_args(ctx);
         A.absD = ctx->d_absD.p;
         A.sqS = ctx->d_sqS.p;
@@ -688,7 +745,7 @@ int mmd_fock_delta(mmd_ctx* ctx, const double* dP_re, const double* dP_im, doubl
         A.D = ctx->d_D.p;
         A.G = ctx->d_G.p;
         HIP_TRY(launch_fill(nn, ctx->d_G.p, 0.0, ctx->st));
-        rc = run_fock_classes(ctx, A);
+        rc = run_fock_classes(ctx, A, tol);
         if (rc) return rc;
         HIP_TRY(launch_symmetrize(N, ctx->d_G.p, ctx->d_Gout.p, ctx->st));
         HIP_TRY(hipMemcpyAsync(G_re, ctx->d_Gout.p, sizeof(double) * nn, k_out, ctx->st));
@@ -704,10 +761,12 @@ int mmd_fock_delta(mmd_ctx* ctx, const double* dP_re, const double* dP_im, doubl
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->st));
     if (nsurv && !raw) {
         int h[NQCLASS];
+        unsigned long long hq[NQCLASS];
         HIP_TRY(hipMemcpyAsync(h, ctx->d_count.p, sizeof(int) * NQCLASS, hipMemcpyDeviceToHost, ctx->st));
+        HIP_TRY(hipMemcpyAsync(hq, ctx->d_nprim.p + NQCLASS, sizeof(hq), hipMemcpyDeviceToHost, ctx->st));
         HIP_TRY(hipStreamSynchronize(ctx->st));
         int64_t s = 0;
-        for (int i = 0; i < NQCLASS; ++i) s += h[i];
+        for (int i = 0; i < NQCLASS; ++i) s += (int64_t)h[i] + (int64_t)hq[i];
         *nsurv = s;
     } else if (nsurv) {
         *nsurv = -1;

@@ -131,8 +131,9 @@ __device__ inline void boys_seeds(double T, double alpha, double pref, const dou
             for (int m = L - 1; m >= 0; --m) Fm[m] = (t2 * Fm[m + 1] + e) * (1.0 / (2.0 * m + 1.0));
         }
     } else {
-        const double inv2T = 0.5 / T;
-        Fm[0] = 0.886226925452758013649083741671 / sqrt(T);
+        const double rT = rsqrt(T);
+        const double inv2T = 0.5 * rT * rT;
+        Fm[0] = 0.886226925452758013649083741671 * rT;
         if (L > 0) {
             const double e = exp(-T);
 #pragma unroll

@@ -8,10 +8,17 @@
 
 namespace mmd {
 
-hipError_t MMD_NAME(fock_class_launch_, QC_LA, QC_LB, QC_LC, QC_LD)(const ClassArgs& A, int grid, hipStream_t st)
+hipError_t MMD_NAME(fock_class_launch_, QC_LA, QC_LB, QC_LC, QC_LD)(const ClassArgs& A, const RowArgs& R, int grid,
+                                                                    hipStream_t st)
 {
-    hipLaunchKernelGGL((eri_fock_class_kernel<QC_LA, QC_LB, QC_LC, QC_LD>), dim3(grid), dim3(256), 0, st, A);
+    const size_t lds = fock_class_lds_bytes<QC_LA, QC_LB, QC_LC, QC_LD>(A.N);
+    hipLaunchKernelGGL((eri_fock_class_kernel<QC_LA, QC_LB, QC_LC, QC_LD>), dim3(grid), dim3(256), lds, st, A, R);
     return hipGetLastError();
+}
+
+size_t MMD_NAME(fock_class_lds_, QC_LA, QC_LB, QC_LC, QC_LD)(int N)
+{
+    return fock_class_lds_bytes<QC_LA, QC_LB, QC_LC, QC_LD>(N);
 }
 
 int MMD_NAME(fock_class_nchunk_, QC_LA, QC_LB, QC_LC, QC_LD)()

@@ -9,21 +9,33 @@ namespace mmd {
     X(2, 0, 1, 0) X(2, 0, 1, 1) X(2, 0, 2, 0) X(2, 1, 0, 0) X(2, 1, 1, 0) X(2, 1, 1, 1) X(2, 1, 2, 0)   \
     X(2, 1, 2, 1) X(2, 2, 0, 0) X(2, 2, 1, 0) X(2, 2, 1, 1) X(2, 2, 2, 0) X(2, 2, 2, 1)
 
-#define DECL(a, b, c, d)                                                                              \
-    hipError_t fock_class_launch_##a##b##c##d(const ClassArgs& A, int grid, hipStream_t st);         \
-    int fock_class_nchunk_##a##b##c##d();
+#define DECL(a, b, c, d)                                                                                       \
+    hipError_t fock_class_launch_##a##b##c##d(const ClassArgs& A, const RowArgs& R, int grid, hipStream_t st); \
+    int fock_class_nchunk_##a##b##c##d();                                                                     \
+    size_t fock_class_lds_##a##b##c##d(int N);
 MMD_CLASSES(DECL)
 #undef DECL
 
-hipError_t launch_fock_class(int qc, const ClassArgs& A, int grid, hipStream_t st)
+hipError_t launch_fock_class(int qc, const ClassArgs& A, const RowArgs& R, int grid, hipStream_t st)
 {
     int la, lb, lc, ld;
     qclass_decode(qc, &la, &lb, &lc, &ld);
 #define CASE(a, b, c, d) \
-    if (la == a && lb == b && lc == c && ld == d) return fock_class_launch_##a##b##c##d(A, grid, st);
+    if (la == a && lb == b && lc == c && ld == d) return fock_class_launch_##a##b##c##d(A, R, grid, st);
     MMD_CLASSES(CASE)
 #undef CASE
     return hipErrorInvalidValue;  // wavefront-per-quartet classes are launched separately
+}
+
+size_t fock_class_lds(int qc, int N)
+{
+    int la, lb, lc, ld;
+    qclass_decode(qc, &la, &lb, &lc, &ld);
+#define CASE(a, b, c, d) \
+    if (la == a && lb == b && lc == c && ld == d) return fock_class_lds_##a##b##c##d(N);
+    MMD_CLASSES(CASE)
+#undef CASE
+    return 0;
 }
 
 int fock_class_nchunk(int qc)

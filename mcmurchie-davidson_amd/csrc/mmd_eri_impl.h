@@ -65,22 +65,24 @@ __device__ __forceinline__ double dmax6(double a, double b, double c, double d, 
 // ---------------------------------------------------------------------------
 // One (quartet, ket-chunk) work item.
 // ---------------------------------------------------------------------------
+// rowA / rowB: rows of G for the bra shells' functions (row x of shell a at
+// rowA + x*N), either in LDS (accumulated per bra pair, flushed later) or
+// the rows of G itself.  The ket-side Coulomb block always goes to G.
 template <int LA, int LB, int LC, int LD, int C0, int C1>
-__device__ __forceinline__ void quartet_item(const ClassArgs& A, const double* lds, int2 qd)
+__device__ __forceinline__ void quartet_item(const ClassArgs& A, const double* lds, int2 qd, const ShellPair& bra,
+                                             const ShellPair& ket, double* rowA, double* rowB)
 {
     using S = ClassShape<LA, LB, LC, LD>;
     constexpr int NCH = C1 - C0;
     constexpr int L = S::L;
     constexpr int HAB = S::HAB;
-    const ShellPair bra = A.pairs[qd.x];
-    const ShellPair ket = A.pairs[qd.y];
 
     double acc[S::NAB * NCH];
     sfor<S::NAB * NCH>([&](auto k) { acc[k] = 0.0; });
 
     for (int ip = 0; ip < bra.npp; ++ip) {
         const PrimPair* P = A.pp + bra.pp0 + ip;
-        const double Pp = P->p, PK = P->K, Px = P->Px, Py = P->Py, Pz = P->Pz;
+        const double Pp = P->p, PK = TWO_PI_2_5 * P->K * P->invp, Px = P->Px, Py = P->Py, Pz = P->Pz;
         double W[HAB * NCH];
         sfor<HAB * NCH>([&](auto k) { W[k] = 0.0; });
         for (int jq = 0; jq < ket.npp; ++jq) {
@@ -96,13 +98,12 @@ __device__ __forceinline__ void quartet_item(const ClassArgs& A, const double* l
                 Ekx[0] = Eky[0] = Ekz[0] = 1.0;
             }
             const double Qp = Q->p;
-            const double pq = Pp * Qp;
-            const double ppq = Pp + Qp;
-            const double rs = 1.0 / sqrt(ppq);
-            const double alpha = pq * rs * rs;
+            const double rs = rsqrt(Pp + Qp);
+            const double alpha = Pp * Qp * rs * rs;
             const double X = Px - Q->Px, Y = Py - Q->Py, Z = Pz - Q->Pz;
             const double T = alpha * (X * X + Y * Y + Z * Z);
-            const double pref = TWO_PI_2_5 / pq * rs * PK * Q->K;
+            // 2 pi^{5/2} / (p q sqrt(p+q)) K_ab K_cd
+            const double pref = PK * (Q->K * Q->invp) * rs;
             double F[L + 1];
             boys_seeds<L>(T, alpha, pref, lds, F);
             double R[nherm(L)];
@@ -218,7 +219,7 @@ __device__ __forceinline__ void quartet_item(const ClassArgs& A, const double* l
             constexpr int c = cd / S::ND, d = cd % S::ND;
             s = fma(acc[ab * NCH + decltype(ic)::value], D[(size_t)(bfc + c) * N + bfd + d], s);
         });
-        atomicAdd(&G[(size_t)(bfa + a) * N + bfb + b], s);
+        atomicAdd(&rowA[(size_t)a * N + bfb + b], s);
     });
     // Coulomb: G_cd += sum_ab (ab|cd) D_ab
     sfor<NCH>([&](auto ic) {
@@ -252,7 +253,7 @@ __device__ __forceinline__ void quartet_item(const ClassArgs& A, const double* l
                     }
                 });
             });
-            if (any) atomicAdd(&G[(size_t)(bfb + x) * N + bfd + y], -0.25 * s);
+            if (any) atomicAdd(&rowB[(size_t)x * N + bfd + y], -0.25 * s);
         });
         sfor<S::NC>([&](auto iy) {        // (b,c) block
             constexpr int y = decltype(iy)::value;
@@ -269,7 +270,7 @@ __device__ __forceinline__ void quartet_item(const ClassArgs& A, const double* l
                     }
                 });
             });
-            if (any) atomicAdd(&G[(size_t)(bfb + x) * N + bfc + y], -0.25 * s);
+            if (any) atomicAdd(&rowB[(size_t)x * N + bfc + y], -0.25 * s);
         });
     });
     sfor<S::NA>([&](auto ix) {
@@ -289,7 +290,7 @@ __device__ __forceinline__ void quartet_item(const ClassArgs& A, const double* l
                     }
                 });
             });
-            if (any) atomicAdd(&G[(size_t)(bfa + x) * N + bfc + y], -0.25 * s);
+            if (any) atomicAdd(&rowA[(size_t)x * N + bfc + y], -0.25 * s);
         });
         sfor<S::ND>([&](auto iy) {        // (a,d) block
             constexpr int y = decltype(iy)::value;
@@ -306,7 +307,7 @@ __device__ __forceinline__ void quartet_item(const ClassArgs& A, const double* l
                     }
                 });
             });
-            if (any) atomicAdd(&G[(size_t)(bfa + x) * N + bfd + y], -0.25 * s);
+            if (any) atomicAdd(&rowA[(size_t)x * N + bfd + y], -0.25 * s);
         });
     });
 }
@@ -321,31 +322,102 @@ __device__ __forceinline__ void static_switch(int i, F&& f)
 }
 
 // ---------------------------------------------------------------------------
-// Class kernel: persistent grid-stride loop over (64-quartet block, chunk).
+// Class kernel.  A work unit is one bra shell pair ("row") and a segment of
+// its ket pairs; workgroups take units from a device counter.  The unit's
+// ket pairs are screened; survivors are compacted in LDS and processed as
+// (64-quartet block, ket chunk) wave items.  Coulomb
+// G_ab and the four exchange blocks, all in the rows of the bra shells, are
+// accumulated in LDS (ds_add_f64) and their nonzero entries added to G once
+// per unit; only the
+// ket-side Coulomb block is added to G directly.  (The host sends a class
+// whose bra rows do not fit in LDS to the wavefront-per-quartet kernel.)
 // ---------------------------------------------------------------------------
+constexpr int ROW_SEG = 1024;   // maximum ket pairs of one work unit
+
 template <int LA, int LB, int LC, int LD>
-__global__ void __launch_bounds__(256) eri_fock_class_kernel(ClassArgs A)
+__global__ void __launch_bounds__(256) eri_fock_class_kernel(ClassArgs A, RowArgs R)
 {
     using S = ClassShape<LA, LB, LC, LD>;
-    __shared__ double lds[BOYS_TAYLOR * BOYS_NT];
-    boys_stage_lds<S::L>(lds, A.boys);
-    __syncthreads();
-    const int nq = *A.count;
-    const int lane = threadIdx.x & 63;
-    const long nblk = ((long)nq + 63) / 64;
-    const long total = nblk * S::NCHUNK;
-    const long wave0 = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const long nwaves = ((long)gridDim.x * blockDim.x) >> 6;
-    for (long w = wave0; w < total; w += nwaves) {
-        const int chunk = (int)(w % S::NCHUNK);
-        const long q = (w / S::NCHUNK) * 64 + lane;
-        if (q >= nq) continue;
-        const int2 qd = A.quart[q];
-        static_switch<0, S::NCHUNK>(chunk, [&](auto ic) {
-            constexpr int C = decltype(ic)::value;
-            quartet_item<LA, LB, LC, LD, C * S::CH, (C + 1) * S::CH>(A, lds, qd);
-        });
+    extern __shared__ double dyn[];
+    double* lds_boys = dyn;
+    double* rows = dyn + BOYS_TAYLOR * BOYS_NT;
+    int* qlist = reinterpret_cast<int*>(rows + (S::NA + S::NB) * A.N);
+    int* misc = qlist + ROW_SEG;
+    boys_stage_lds<S::L>(lds_boys, A.boys);
+    const int N = A.N;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int upr = (R.nk + R.seg - 1) / R.seg;          // units per row
+    const int nunits = R.nb * upr;
+    const size_t ns = (size_t)R.nshell;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) misc[0] = atomicAdd(R.row_counter, 1);
+        __syncthreads();
+        const int u = misc[0];
+        if (u >= nunits) break;
+        // triangular classes hand out their long rows first
+        const int P = R.tri ? (R.nb - 1 - u / upr) : u / upr;
+        const int k0 = (u % upr) * R.seg;
+        const int nk = R.tri ? P + 1 : R.nk;
+        if (k0 >= nk) continue;
+        const int segn = min(R.seg, nk - k0);
+        const ShellPair bra = A.pairs[R.boff + P];
+        const int nrows = bra.same ? S::NA : S::NA + S::NB;
+        for (int i = threadIdx.x; i < nrows * N; i += blockDim.x) rows[i] = 0.0;
+        if (threadIdx.x == 0) misc[1] = 0;
+        __syncthreads();
+        for (int t0 = 0; t0 < segn; t0 += blockDim.x) {
+            const int t = t0 + threadIdx.x;
+            bool keep = false;
+            if (t < segn) {
+                const ShellPair ket = A.pairs[R.koff + k0 + t];
+                const double* Dsh = R.Dsh;
+                const double dm = fmax(fmax(fmax(4.0 * Dsh[bra.sa * ns + bra.sb], 4.0 * Dsh[ket.sa * ns + ket.sb]),
+                                            fmax(Dsh[bra.sa * ns + ket.sa], Dsh[bra.sa * ns + ket.sb])),
+                                       fmax(Dsh[bra.sb * ns + ket.sa], Dsh[bra.sb * ns + ket.sb]));
+                keep = bra.q * ket.q * dm >= A.tol;
+            }
+            const unsigned long long m = __ballot(keep);
+            int pos = 0;
+            if (lane == 0 && m) pos = atomicAdd(&misc[1], __popcll(m));
+            pos = __shfl(pos, 0);
+            if (keep) qlist[pos + __popcll(m & ((1ull << lane) - 1ull))] = k0 + t;
+        }
+        __syncthreads();
+        const int nsurv = misc[1];
+        if (nsurv == 0) continue;
+        if (R.stat_nquart && threadIdx.x == 0) atomicAdd(R.stat_nquart, (unsigned long long)nsurv);
+        const int total = ((nsurv + 63) / 64) * S::NCHUNK;
+        for (int w = wv; w < total; w += (int)(blockDim.x >> 6)) {
+            const int chunk = w % S::NCHUNK;
+            const int idx = (w / S::NCHUNK) * 64 + lane;
+            if (idx >= nsurv) continue;
+            const int Q = qlist[idx];
+            const int2 qd = make_int2(R.boff + P, R.koff + Q);
+            const ShellPair ket = A.pairs[qd.y];
+            if (R.stat_nprim && chunk == 0)
+                atomicAdd(R.stat_nprim, (unsigned long long)bra.npp * (unsigned long long)ket.npp);
+            static_switch<0, S::NCHUNK>(chunk, [&](auto ic) {
+                constexpr int C = decltype(ic)::value;
+                quartet_item<LA, LB, LC, LD, C * S::CH, (C + 1) * S::CH>(
+                    A, lds_boys, qd, bra, ket, rows, rows + (bra.same ? 0 : S::NA * N));
+            });
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < nrows * N; i += blockDim.x) {
+            const double v = rows[i];
+            const int x = i / N, col = i - x * N;
+            const int g = x < S::NA ? bra.bfa + x : bra.bfb + (x - S::NA);
+            if (v != 0.0) atomicAdd(&A.G[(size_t)g * N + col], v);
+        }
     }
+}
+
+template <int LA, int LB, int LC, int LD>
+inline size_t fock_class_lds_bytes(int N)
+{
+    using S = ClassShape<LA, LB, LC, LD>;
+    return sizeof(double) * (BOYS_TAYLOR * BOYS_NT + (size_t)(S::NA + S::NB) * N) + sizeof(int) * (ROW_SEG + 4);
 }
 
 }  // namespace mmd

@@ -18,13 +18,14 @@ struct ShellPair {
 };
 
 // A primitive pair: exponent sum, Gaussian-product prefactor including both
-// contraction coefficients, product centre and its offsets from A and B.
+// contraction coefficients, product centre and its offsets from A and B,
+// 1/(2p) and 1/p.
 struct PrimPair {
     double p, K;
     double Px, Py, Pz;
     double PAx, PAy, PAz;
     double PBx, PBy, PBz;
-    double inv2p, pad;
+    double inv2p, invp;
 };
 
 // Arguments of the quartet-class kernels (passed by value).
@@ -41,6 +42,19 @@ struct ClassArgs {
     double tol;                 // screening threshold
     int N;
     double* G;                  // Fock accumulation N x N
+};
+
+// Row decomposition of one quartet class for the Fock kernels.
+struct RowArgs {
+    int boff, nb;               // bra pairs of the class: pairs[boff .. boff+nb)
+    int koff, nk;               // ket pairs of the class
+    int tri;                    // same pair class: ket index <= bra index
+    int seg;                    // ket pairs per work unit
+    int nshell;
+    const double* Dsh;          // shell-block maxima of |dP| (nshell x nshell)
+    int* row_counter;           // next work unit (zeroed before the launch)
+    unsigned long long* stat_nquart;   // optional statistics
+    unsigned long long* stat_nprim;
 };
 
 

@@ -489,13 +489,175 @@ __global__ void __launch_bounds__(64 * GW_WAVES) fock_wave_class_kernel(ClassArg
     }
 }
 
+// Compile-time variant for one class (used for (dd|dd)): lane m evaluates
+// F_m, the R_tuv levels are computed in parallel across the wavefront
+// through two LDS buffers, the E tables by six lanes; the ket contraction
+// W_tuv(cd) is spread over the lanes into LDS, then every lane applies the
+// bra expansion to its component quartets.
+template <int LA, int LB, int LC, int LD>
+__global__ void __launch_bounds__(64 * GW_WAVES) fock_wave_class_kernel_t(ClassArgs A)
+{
+    constexpr int L = LA + LB + LC + LD;
+    constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    constexpr int NAB = NA * NB, NCD = NC * ND, NTOT = NAB * NCD;
+    constexpr int NCPL = (NTOT + 63) / 64;
+    constexpr int HL = nherm(L);
+    constexpr int NTB = LA + LB + 1, NTK = LC + LD + 1;
+    constexpr int EB = (LA + 1) * (LB + 1) * NTB, EK = (LC + 1) * (LD + 1) * NTK;
+    constexpr int HAB = nherm(LA + LB);
+    __shared__ double Ws[GW_WAVES][HAB * NCD];
+    __shared__ double Rb[GW_WAVES][2][HL];
+    __shared__ double Es[GW_WAVES][3 * EB + 3 * EK];
+    __shared__ double Fs[GW_WAVES][L + 1];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nq = *A.count;
+    const int N = A.N;
+    for (int64_t q = (int64_t)blockIdx.x * GW_WAVES + wv; q < nq; q += (int64_t)gridDim.x * GW_WAVES) {
+        const int2 qd = A.quart[q];
+        const ShellPair bra = A.pairs[qd.x];
+        const ShellPair ket = A.pairs[qd.y];
+        double acc[NCPL];
+        for (int i = 0; i < NCPL; ++i) acc[i] = 0.0;
+        for (int ip = 0; ip < bra.npp; ++ip) {
+            const PrimPair* P = A.pp + bra.pp0 + ip;
+            for (int jq = 0; jq < ket.npp; ++jq) {
+                const PrimPair* Q = A.pp + ket.pp0 + jq;
+                const double rs = rsqrt(P->p + Q->p);
+                const double alpha = P->p * Q->p * rs * rs;
+                const double X = P->Px - Q->Px, Y = P->Py - Q->Py, Z = P->Pz - Q->Pz;
+                const double T = alpha * (X * X + Y * Y + Z * Z);
+                if (lane <= L) {
+                    double F[MAX_LTOT + 1];
+                    boys_runtime(lane, T, A.boys, F);
+                    double s = 34.98683665524972497 * P->K * P->invp * Q->K * Q->invp * rs;
+                    for (int m = 0; m < lane; ++m) s *= -2.0 * alpha;
+                    Fs[wv][lane] = s * F[lane];
+                } else if (lane >= 32 && lane < 38) {
+                    const int x = (lane - 32) % 3;
+                    double E[EB > EK ? EB : EK];
+                    if (lane < 35) {
+                        hermite_E<LA, LB, 1>(P->inv2p, x == 0 ? P->PAx : (x == 1 ? P->PAy : P->PAz),
+                                             x == 0 ? P->PBx : (x == 1 ? P->PBy : P->PBz), E);
+                        for (int k = 0; k < EB; ++k) Es[wv][x * EB + k] = E[k];
+                    } else {
+                        hermite_E<LC, LD, -1>(Q->inv2p, x == 0 ? Q->PAx : (x == 1 ? Q->PAy : Q->PAz),
+                                              x == 0 ? Q->PBx : (x == 1 ? Q->PBy : Q->PBz), E);
+                        for (int k = 0; k < EK; ++k) Es[wv][3 * EB + x * EK + k] = E[k];
+                    }
+                }
+                wave_sync();
+                if (lane == 0) Rb[wv][L & 1][0] = Fs[wv][L];
+                wave_sync();
+                for (int n = L - 1; n >= 0; --n) {
+                    const double* prev = Rb[wv][(n + 1) & 1];
+                    double* cur = Rb[wv][n & 1];
+                    for (int k = lane; k < nherm(L - n); k += 64) {
+                        double val;
+                        if (k == 0) {
+                            val = Fs[wv][n];
+                        } else {
+                            const Herm3 h = hdecode(k);
+                            if (h.t > 0) {
+                                val = X * prev[hidx(h.t - 1, h.u, h.v)];
+                                if (h.t > 1) val += (double)(h.t - 1) * prev[hidx(h.t - 2, h.u, h.v)];
+                            } else if (h.u > 0) {
+                                val = Y * prev[hidx(h.t, h.u - 1, h.v)];
+                                if (h.u > 1) val += (double)(h.u - 1) * prev[hidx(h.t, h.u - 2, h.v)];
+                            } else {
+                                val = Z * prev[hidx(h.t, h.u, h.v - 1)];
+                                if (h.v > 1) val += (double)(h.v - 1) * prev[hidx(h.t, h.u, h.v - 2)];
+                            }
+                        }
+                        cur[k] = val;
+                    }
+                    wave_sync();
+                }
+                const double* R = Rb[wv][0];
+                const double* Eb = Es[wv];
+                const double* Ek = Es[wv] + 3 * EB;
+                // ket contraction into the bra Hermite space: W[k][cd]
+                double* W = Ws[wv];
+                for (int e = lane; e < HAB * NCD; e += 64) {
+                    const int k = e / NCD, cd = e % NCD;
+                    const Herm3 h = hdecode(k);
+                    const int c = cd / ND, d = cd % ND;
+                    const int cx = comp_lx(LC, c), cy = comp_ly(LC, c), cz = comp_lz(LC, c);
+                    const int dx = comp_lx(LD, d), dy = comp_ly(LD, d), dz = comp_lz(LD, d);
+                    const double* ekx = Ek + (cx * (LD + 1) + dx) * NTK;
+                    const double* eky = Ek + EK + (cy * (LD + 1) + dy) * NTK;
+                    const double* ekz = Ek + 2 * EK + (cz * (LD + 1) + dz) * NTK;
+                    double w = 0.0;
+                    for (int tau = 0; tau <= cx + dx; ++tau)
+                        for (int nu = 0; nu <= cy + dy; ++nu) {
+                            const double e2 = ekx[tau] * eky[nu];
+                            for (int phi = 0; phi <= cz + dz; ++phi)
+                                w = fma(e2 * ekz[phi], R[hidx(h.t + tau, h.u + nu, h.v + phi)], w);
+                        }
+                    W[e] = w;
+                }
+                wave_sync();
+                // bra expansion
+#pragma unroll
+                for (int i = 0; i < NCPL; ++i) {
+                    const int k = lane + 64 * i;
+                    if (k < NTOT) {
+                        const int ab = k / NCD, cd = k % NCD;
+                        const int a = ab / NB, b = ab % NB;
+                        const int ax = comp_lx(LA, a), ay = comp_ly(LA, a), az = comp_lz(LA, a);
+                        const int bx = comp_lx(LB, b), by = comp_ly(LB, b), bz = comp_lz(LB, b);
+                        const double* ebx = Eb + (ax * (LB + 1) + bx) * NTB;
+                        const double* eby = Eb + EB + (ay * (LB + 1) + by) * NTB;
+                        const double* ebz = Eb + 2 * EB + (az * (LB + 1) + bz) * NTB;
+                        double v = 0.0;
+                        for (int t = 0; t <= ax + bx; ++t)
+                            for (int u = 0; u <= ay + by; ++u) {
+                                const double exy = ebx[t] * eby[u];
+                                for (int w = 0; w <= az + bz; ++w)
+                                    v = fma(exy * ebz[w], W[hidx(t, u, w) * NCD + cd], v);
+                            }
+                        acc[i] += v;
+                    }
+                }
+                wave_sync();
+            }
+        }
+        const int bfa = bra.bfa, bfb = bra.bfb, bfc = ket.bfa, bfd = ket.bfb;
+        const double deg = (bra.same ? 1.0 : 2.0) * (ket.same ? 1.0 : 2.0) * (qd.x == qd.y ? 1.0 : 2.0);
+#pragma unroll
+        for (int i = 0; i < NCPL; ++i) {
+            const int k = lane + 64 * i;
+            if (k >= NTOT) continue;
+            const int ab = k / NCD, cd = k % NCD;
+            const int p = bfa + ab / NB, qq = bfb + ab % NB, r = bfc + cd / ND, s = bfd + cd % ND;
+            const double* aD = A.absD;
+            const double bound = A.sqS[(size_t)p * N + qq] * A.sqS[(size_t)r * N + s]
+                * fmax(fmax(fmax(4.0 * aD[(size_t)p * N + qq], 4.0 * aD[(size_t)r * N + s]),
+                            fmax(aD[(size_t)p * N + r], aD[(size_t)p * N + s])),
+                       fmax(aD[(size_t)qq * N + r], aD[(size_t)qq * N + s]));
+            if (bound < A.tol) continue;
+            const double v = acc[i] * deg * A.gcomp[p] * A.gcomp[qq] * A.gcomp[r] * A.gcomp[s];
+            const double* D = A.D;
+            double* G = A.G;
+            atomicAdd(&G[(size_t)p * N + qq], v * D[(size_t)r * N + s]);
+            atomicAdd(&G[(size_t)r * N + s], v * D[(size_t)p * N + qq]);
+            atomicAdd(&G[(size_t)qq * N + s], -0.25 * v * D[(size_t)p * N + r]);
+            atomicAdd(&G[(size_t)qq * N + r], -0.25 * v * D[(size_t)p * N + s]);
+            atomicAdd(&G[(size_t)p * N + r], -0.25 * v * D[(size_t)qq * N + s]);
+            atomicAdd(&G[(size_t)p * N + s], -0.25 * v * D[(size_t)qq * N + r]);
+        }
+    }
+}
+
 hipError_t launch_fock_wave_class(int la, int lb, int lc, int ld, const ClassArgs& A, int64_t max_quartets,
                                   hipStream_t st)
 {
     int64_t grid = (max_quartets + GW_WAVES - 1) / GW_WAVES;
     if (grid < 1) grid = 1;
     if (grid > 4096) grid = 4096;
-    hipLaunchKernelGGL(fock_wave_class_kernel, dim3((unsigned)grid), dim3(64 * GW_WAVES), 0, st, A, la, lb, lc, ld);
+    if (la == 2 && lb == 2 && lc == 2 && ld == 2)
+        hipLaunchKernelGGL((fock_wave_class_kernel_t<2, 2, 2, 2>), dim3((unsigned)grid), dim3(64 * GW_WAVES), 0, st, A);
+    else
+        hipLaunchKernelGGL(fock_wave_class_kernel, dim3((unsigned)grid), dim3(64 * GW_WAVES), 0, st, A, la, lb, lc, ld);
     return hipGetLastError();
 }
 

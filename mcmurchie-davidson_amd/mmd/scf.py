@@ -7,10 +7,23 @@ computes the symmetrised incremental G(P - P_old) with the GPU Fock kernels
 (formPT semantics, cython/fock.pyx:10-87), the conventional path contracts
 the stored tensor on the GPU (mmd/scf.py:97-98).
 """
+import contextlib
+
 import numpy as np
 import scipy.linalg
 
 from mmd.integrals.fock import formPT
+
+try:  # small dense algebra (N <= a few hundred) is fastest on one BLAS thread
+    from threadpoolctl import threadpool_limits as _threadpool_limits
+except ImportError:  # pragma: no cover
+    _threadpool_limits = None
+
+
+def _blas_threads(n):
+    if _threadpool_limits is None:
+        return contextlib.nullcontext()
+    return _threadpool_limits(limits=n, user_api="blas")
 
 
 def _re(A):
@@ -28,6 +41,10 @@ class SCF(object):
     def RHF(self, doPrint=True, DIIS=True, direct=False, conver=1e-8, acc2e=1e-12):
         """Closed-shell SCF (mmd/scf.py:10-80).  Converged when
         ||P - P_old||_F < conver; at most 100 iterations."""
+        with _blas_threads(1 if self.nbasis < 600 else 4):
+            return self._rhf(doPrint, DIIS, direct, conver, acc2e)
+
+    def _rhf(self, doPrint, DIIS, direct, conver, acc2e):
         self.is_converged = False
         self.delta_energy = 1e20
         self.P_RMS = 1e20

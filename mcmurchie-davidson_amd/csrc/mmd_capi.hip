@@ -370,10 +370,14 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol)
     HIP_TRY(launch_fill_int(NQCLASS, c->d_count.p, 0, c->st));
     HIP_TRY(launch_fill_int(4 * NQCLASS, reinterpret_cast<int*>(c->d_nprim.p), 0, c->st));
     bool row_kernel[NQCLASS];
+    int maxpp[NQCLASS];
     for (int qc = 0; qc < NQCLASS; ++qc) {
-        int la, lb, lc, ld;
+        int la, lb, lc, ld, pcb, pck;
         qclass_decode(qc, &la, &lb, &lc, &ld);
-        row_kernel[qc] = !fock_class_is_wave(la, lb, lc, ld) && fock_class_lds(qc, c->nbf) <= c->max_lds;
+        class_of(qc, &pcb, &pck);
+        maxpp[qc] = 1;
+        for (int P = c->pc_off[pcb]; P < c->pc_off[pcb + 1]; ++P) maxpp[qc] = std::max(maxpp[qc], c->pairs[P].npp);
+        row_kernel[qc] = !fock_class_is_wave(la, lb, lc, ld) && fock_class_lds(qc, c->nbf, maxpp[qc]) <= c->max_lds;
     }
     for (int qc = 0; qc < NQCLASS; ++qc) {
         if (c->qc_total[qc] == 0) continue;
@@ -393,12 +397,12 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol)
             R.nshell = (int)c->shells.size();
             R.Dsh = c->d_Dsh.p;
             R.row_counter = c->d_rowctr.p + qc;
-            // ket segment: enough kets for the workgroup's four waves, short
-            // enough for a few thousand work units per class
-            const int nch = fock_class_nchunk(qc);
-            int seg = ((4 + nch - 1) / nch) * 64;
+            R.maxpp = maxpp[qc];
+            // ket segment: long (survivors are batched inside a unit), but
+            // short enough for a few thousand work units per class
             const int64_t quartets = R.tri ? (int64_t)nb * (nb + 1) / 2 : (int64_t)nb * nk;
-            while (seg > 64 && quartets / seg < 4096) seg -= 64;
+            int seg = 4096;
+            while (seg > 256 && quartets / seg < 4096) seg /= 2;
             R.seg = seg;
             const int64_t units = (int64_t)nb * ((nk + seg - 1) / seg);
             R.stat_nquart = c->d_nprim.p + NQCLASS + qc;

@@ -11,14 +11,14 @@ namespace mmd {
 hipError_t MMD_NAME(fock_class_launch_, QC_LA, QC_LB, QC_LC, QC_LD)(const ClassArgs& A, const RowArgs& R, int grid,
                                                                     hipStream_t st)
 {
-    const size_t lds = fock_class_lds_bytes<QC_LA, QC_LB, QC_LC, QC_LD>(A.N);
+    const size_t lds = fock_class_lds_bytes<QC_LA, QC_LB, QC_LC, QC_LD>(A.N, R.maxpp);
     hipLaunchKernelGGL((eri_fock_class_kernel<QC_LA, QC_LB, QC_LC, QC_LD>), dim3(grid), dim3(256), lds, st, A, R);
     return hipGetLastError();
 }
 
-size_t MMD_NAME(fock_class_lds_, QC_LA, QC_LB, QC_LC, QC_LD)(int N)
+size_t MMD_NAME(fock_class_lds_, QC_LA, QC_LB, QC_LC, QC_LD)(int N, int maxpp)
 {
-    return fock_class_lds_bytes<QC_LA, QC_LB, QC_LC, QC_LD>(N);
+    return fock_class_lds_bytes<QC_LA, QC_LB, QC_LC, QC_LD>(N, maxpp);
 }
 
 int MMD_NAME(fock_class_nchunk_, QC_LA, QC_LB, QC_LC, QC_LD)()

@@ -12,7 +12,7 @@ namespace mmd {
 #define DECL(a, b, c, d)                                                                                       \
     hipError_t fock_class_launch_##a##b##c##d(const ClassArgs& A, const RowArgs& R, int grid, hipStream_t st); \
     int fock_class_nchunk_##a##b##c##d();                                                                     \
-    size_t fock_class_lds_##a##b##c##d(int N);
+    size_t fock_class_lds_##a##b##c##d(int N, int maxpp);
 MMD_CLASSES(DECL)
 #undef DECL
 
@@ -27,12 +27,12 @@ hipError_t launch_fock_class(int qc, const ClassArgs& A, const RowArgs& R, int g
     return hipErrorInvalidValue;  // wavefront-per-quartet classes are launched separately
 }
 
-size_t fock_class_lds(int qc, int N)
+size_t fock_class_lds(int qc, int N, int maxpp)
 {
     int la, lb, lc, ld;
     qclass_decode(qc, &la, &lb, &lc, &ld);
 #define CASE(a, b, c, d) \
-    if (la == a && lb == b && lc == c && ld == d) return fock_class_lds_##a##b##c##d(N);
+    if (la == a && lb == b && lc == c && ld == d) return fock_class_lds_##a##b##c##d(N, maxpp);
     MMD_CLASSES(CASE)
 #undef CASE
     return 0;

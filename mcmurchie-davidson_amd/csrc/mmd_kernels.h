@@ -23,7 +23,7 @@ inline __host__ __device__ void qclass_decode(int qc, int* la, int* lb, int* lc,
 }
 
 hipError_t launch_fock_class(int qc, const ClassArgs& A, const RowArgs& R, int grid, hipStream_t st);
-size_t fock_class_lds(int qc, int N);
+size_t fock_class_lds(int qc, int N, int maxpp);
 hipError_t launch_fock_wave_class(int la, int lb, int lc, int ld, const ClassArgs& A, int64_t max_quartets,
                                   hipStream_t st);
 // Classes served by the wavefront-per-quartet kernel instead of a unrolled class kernel.

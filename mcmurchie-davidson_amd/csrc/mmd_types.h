@@ -50,6 +50,7 @@ struct RowArgs {
     int koff, nk;               // ket pairs of the class
     int tri;                    // same pair class: ket index <= bra index
     int seg;                    // ket pairs per work unit
+    int maxpp;                  // largest primitive-pair count of the class's bra pairs
     int nshell;
     const double* Dsh;          // shell-block maxima of |dP| (nshell x nshell)
     int* row_counter;           // next work unit (zeroed before the launch)

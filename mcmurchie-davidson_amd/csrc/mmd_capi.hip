@@ -77,6 +77,11 @@ struct mmd_ctx {
     int device = 0;
     hipStream_t st = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // side streams for concurrent class kernels (default priority), with the
+    // fork / join events that order them against st
+    static constexpr int NSIDE = 3;
+    hipStream_t side[NSIDE] = {};
+    hipEvent_t evfork = nullptr, evjoin[NSIDE] = {};
     double last_fock_ms = 0.0;
 
     // basis, per function
@@ -379,14 +384,49 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol)
         for (int P = c->pc_off[pcb]; P < c->pc_off[pcb + 1]; ++P) maxpp[qc] = std::max(maxpp[qc], c->pairs[P].npp);
         row_kernel[qc] = !fock_class_is_wave(la, lb, lc, ld) && fock_class_lds(qc, c->nbf, maxpp[qc]) <= c->max_lds;
     }
+    // Classes run concurrently on the main and side streams (longest first,
+    // each to the least loaded stream), so that the small classes fill the
+    // GPU beside the large ones.  With profiling on they run one after the
+    // other on the main stream, so each class's events time it alone.
+    double pc_npp[NPAIRCLASS] = {0};
+    for (int pc = 0; pc < NPAIRCLASS; ++pc)
+        for (int P = c->pc_off[pc]; P < c->pc_off[pc + 1]; ++P) pc_npp[pc] += c->pairs[P].npp;
+    int order[NQCLASS], nord = 0;
+    double cost[NQCLASS] = {0};
     for (int qc = 0; qc < NQCLASS; ++qc) {
         if (c->qc_total[qc] == 0) continue;
         int pcb, pck, la, lb, lc, ld;
         class_of(qc, &pcb, &pck);
         qclass_decode(qc, &la, &lb, &lc, &ld);
+        const double prim = pc_npp[pcb] * pc_npp[pck] * (pcb == pck ? 0.5 : 1.0);
+        const int L = la + lb + lc + ld;
+        cost[qc] = prim * (40.0 + 4.0 * nherm(L) + (double)ncart(la) * ncart(lb) * ncart(lc) * ncart(ld));
+        order[nord++] = qc;
+    }
+    std::sort(order, order + nord, [&](int x, int y) { return cost[x] > cost[y]; });
+    const int nstream = c->profiling ? 1 : 1 + mmd_ctx::NSIDE;
+    hipStream_t streams[1 + mmd_ctx::NSIDE] = {c->st};
+    for (int i = 0; i < mmd_ctx::NSIDE; ++i) streams[1 + i] = c->side[i];
+    double load[1 + mmd_ctx::NSIDE] = {0};
+    bool used[1 + mmd_ctx::NSIDE] = {true};
+    if (nstream > 1) HIP_TRY(hipEventRecord(c->evfork, c->st));
+    for (int oi = 0; oi < nord; ++oi) {
+        const int qc = order[oi];
+        int si = 0;
+        for (int i = 1; i < nstream; ++i)
+            if (load[i] < load[si]) si = i;
+        load[si] += cost[qc];
+        hipStream_t sq = streams[si];
+        if (!used[si]) {
+            HIP_TRY(hipStreamWaitEvent(sq, c->evfork, 0));
+            used[si] = true;
+        }
+        int pcb, pck, la, lb, lc, ld;
+        class_of(qc, &pcb, &pck);
+        qclass_decode(qc, &la, &lb, &lc, &ld);
         const int nb = c->pc_off[pcb + 1] - c->pc_off[pcb];
         const int nk = c->pc_off[pck + 1] - c->pc_off[pck];
-        if (c->profiling) HIP_TRY(hipEventRecord(c->evc[2 * qc], c->st));
+        if (c->profiling) HIP_TRY(hipEventRecord(c->evc[2 * qc], sq));
         if (row_kernel[qc]) {
             RowArgs R;
             R.boff = c->pc_off[pcb];
@@ -407,18 +447,24 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol)
             const int64_t units = (int64_t)nb * ((nk + seg - 1) / seg);
             R.stat_nquart = c->d_nprim.p + NQCLASS + qc;
             R.stat_nprim = c->profiling ? c->d_nprim.p + qc : nullptr;
-            HIP_TRY(launch_fock_class(qc, A, R, (int)std::min<int64_t>(units, 2048), c->st));
+            HIP_TRY(launch_fock_class(qc, A, R, (int)std::min<int64_t>(units, 2048), sq));
         } else {
             HIP_TRY(launch_screen_class(c->pc_off[pcb], nb, c->pc_off[pck], nk, pcb == pck ? 1 : 0, c->d_pairs.p,
                                         c->d_Dsh.p, (int)c->shells.size(), tol, 0, c->d_surv.p + c->qc_off[qc],
-                                        c->d_count.p + qc, c->st));
-            A.quart = c->d_surv.p + c->qc_off[qc];
-            A.count = c->d_count.p + qc;
+                                        c->d_count.p + qc, sq));
+            ClassArgs W = A;
+            W.quart = c->d_surv.p + c->qc_off[qc];
+            W.count = c->d_count.p + qc;
             if (c->profiling)
-                HIP_TRY(launch_count_prims(A.quart, A.count, c->d_pairs.p, c->d_nprim.p + qc, c->st));
-            HIP_TRY(launch_fock_wave_class(la, lb, lc, ld, A, c->qc_total[qc], c->st));
+                HIP_TRY(launch_count_prims(W.quart, W.count, c->d_pairs.p, c->d_nprim.p + qc, sq));
+            HIP_TRY(launch_fock_wave_class(la, lb, lc, ld, W, c->qc_total[qc], sq));
         }
-        if (c->profiling) HIP_TRY(hipEventRecord(c->evc[2 * qc + 1], c->st));
+        if (c->profiling) HIP_TRY(hipEventRecord(c->evc[2 * qc + 1], sq));
+    }
+    for (int i = 1; i < nstream; ++i) {
+        if (!used[i]) continue;
+        HIP_TRY(hipEventRecord(c->evjoin[i - 1], streams[i]));
+        HIP_TRY(hipStreamWaitEvent(c->st, c->evjoin[i - 1], 0));
     }
     if (c->profiling) {
         int h[NQCLASS];
@@ -511,6 +557,11 @@ int mmd_ctx_create(mmd_ctx** out, int device_id)
     hipError_t e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->evfork, hipEventDisableTiming);
+    for (int i = 0; i < mmd_ctx::NSIDE && e == hipSuccess; ++i) {
+        e = hipStreamCreateWithFlags(&c->side[i], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->evjoin[i], hipEventDisableTiming);
+    }
     std::vector<double> tab;
     boys_build_table(tab);
     if (e == hipSuccess) e = c->d_boys.reserve(tab.size());
@@ -532,6 +583,12 @@ void mmd_ctx_destroy(mmd_ctx* ctx)
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     for (int i = 0; i < 2 * NQCLASS; ++i)
         if (ctx->evc[i]) (void)hipEventDestroy(ctx->evc[i]);
+    for (int i = 0; i < mmd_ctx::NSIDE; ++i) {
+        if (ctx->side[i]) (void)hipStreamSynchronize(ctx->side[i]);
+        if (ctx->evjoin[i]) (void)hipEventDestroy(ctx->evjoin[i]);
+        if (ctx->side[i]) (void)hipStreamDestroy(ctx->side[i]);
+    }
+    if (ctx->evfork) (void)hipEventDestroy(ctx->evfork);
     hipStream_t st = ctx->st;
     delete ctx;  // frees device buffers
     if (st) (void)hipStreamDestroy(st);
@@ -645,29 +702,11 @@ int mmd_schwarz(mmd_ctx* ctx, double* diag)
     if (ctx->nbf == 0) return fail(MMD_ERR_STATE, "no basis set");
     const int N = ctx->nbf;
     const size_t nn = (size_t)N * N;
-    const int64_t npair = (int64_t)N * (N + 1) / 2;
-    std::vector<int64_t> idx(4 * npair);
-    int64_t k = 0;
-    for (int p = 0; p < N; ++p)
-        for (int q = 0; q <= p; ++q, ++k) {
-            idx[4 * k] = p;
-            idx[4 * k + 1] = q;
-            idx[4 * k + 2] = p;
-            idx[4 * k + 3] = q;
-        }
-    DevBuf<int64_t> d_idx;
-    DevBuf<double> d_val;
-    HIP_TRY(d_idx.reserve(4 * npair));
-    HIP_TRY(d_val.reserve(npair));
-    HIP_TRY(hipMemcpyAsync(d_idx.p, idx.data(), sizeof(int64_t) * 4 * npair, hipMemcpyHostToDevice, ctx->st));
-    HIP_TRY(launch_eri_bf_list(npair, d_idx.p, ctx->d_bf.p, ctx->d_bf_exp.p, ctx->d_bf_w.p, ctx->d_boys.p, d_val.p,
-                               ctx->st));
-    std::vector<double> v(npair), h(nn);
-    HIP_TRY(hipMemcpyAsync(v.data(), d_val.p, sizeof(double) * npair, hipMemcpyDeviceToHost, ctx->st));
+    std::vector<double> h(nn);
+    HIP_TRY(ctx->d_schwarz.reserve(nn));
+    HIP_TRY(launch_eri_diag(N, ctx->d_bf.p, ctx->d_bf_exp.p, ctx->d_bf_w.p, ctx->d_boys.p, ctx->d_schwarz.p, ctx->st));
+    HIP_TRY(hipMemcpyAsync(h.data(), ctx->d_schwarz.p, sizeof(double) * nn, hipMemcpyDeviceToHost, ctx->st));
     HIP_TRY(hipStreamSynchronize(ctx->st));
-    k = 0;
-    for (int p = 0; p < N; ++p)
-        for (int q = 0; q <= p; ++q, ++k) h[(size_t)p * N + q] = h[(size_t)q * N + p] = v[k];
     if (diag) memcpy(diag, h.data(), sizeof(double) * nn);
     return set_schwarz(ctx, h);
 }

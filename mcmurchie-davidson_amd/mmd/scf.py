@@ -76,7 +76,7 @@ class SCF(object):
             if not DIIS or step == 0:
                 self.orthoFock()
 
-            E, self.CO = scipy.linalg.eigh(_re(self.FO))
+            E, self.CO = scipy.linalg.eigh(_re(self.FO), driver="evd")
             C = np.dot(self.X, self.CO)
             self.C = C
             self.MO = E

@@ -40,6 +40,8 @@ struct BFDesc {
     int32_t pad;
 };
 
+hipError_t launch_eri_diag(int N, const BFDesc* bfs, const double* pexp, const double* pw, const double* boys,
+                           double* out, hipStream_t st);
 hipError_t launch_eri_bf_list(int64_t nq, const int64_t* ijkl, const BFDesc* bfs, const double* pexp,
                               const double* pw, const double* boys, double* out, hipStream_t st);
 hipError_t launch_canonical_quartets(int mode, int N, const BFDesc* bfs, const double* pexp, const double* pw,

@@ -14,69 +14,127 @@ namespace mmd {
 // entry point, the Schwarz diagonal, the stored tensor (doERIs) and formPT's
 // unsymmetrised output; the Fock build itself uses the class kernels.
 // ---------------------------------------------------------------------------
+// Hermite expansion of one primitive pair: centre, Gaussian-product weight
+// (times the contraction weights) and the three 1-D E tables.
+struct PrimPairRT {
+    double p, w, P[3], E[3][45];
+};
+
+__device__ inline void prim_pair_rt(const BFDesc& a, const BFDesc& b, int ia, int ib, const double* __restrict__ pexp,
+                                    const double* __restrict__ pw, PrimPairRT& o)
+{
+    const double A[3] = {a.x, a.y, a.z}, B[3] = {b.x, b.y, b.z};
+    const int la[3] = {a.l, a.m, a.n}, lb[3] = {b.l, b.m, b.n};
+    const double ea = pexp[a.p0 + ia], eb = pexp[b.p0 + ib];
+    o.p = ea + eb;
+    double k = 1.0;
+    for (int x = 0; x < 3; ++x) {
+        o.P[x] = (ea * A[x] + eb * B[x]) / o.p;
+        k *= exp(-ea * eb / o.p * (A[x] - B[x]) * (A[x] - B[x]));
+        etab_rt(la[x], lb[x], 0.5 / o.p, o.P[x] - A[x], o.P[x] - B[x], o.E[x]);
+    }
+    o.w = pw[a.p0 + ia] * pw[b.p0 + ib] * k;
+}
+
+// (ab|cd) of one primitive quartet (cython/twoe.pyx:52-94).
+__device__ inline double prim_quartet_rt(const BFDesc& a, const BFDesc& b, const BFDesc& c, const BFDesc& d,
+                                         const PrimPairRT& P, const PrimPairRT& Q, const double* __restrict__ boys)
+{
+    const int L = a.l + a.m + a.n + b.l + b.m + b.n + c.l + c.m + c.n + d.l + d.m + d.n;
+    const int la[3] = {a.l, a.m, a.n}, lb[3] = {b.l, b.m, b.n}, lc[3] = {c.l, c.m, c.n}, ld[3] = {d.l, d.m, d.n};
+    const int nbx = la[0] + lb[0] + 1, nby = la[1] + lb[1] + 1, nbz = la[2] + lb[2] + 1;
+    const int nkx = lc[0] + ld[0] + 1, nky = lc[1] + ld[1] + 1, nkz = lc[2] + ld[2] + 1;
+    const double p = P.p, qq = Q.p;
+    const double alpha = p * qq / (p + qq);
+    const double X = P.P[0] - Q.P[0], Y = P.P[1] - Q.P[1], Z = P.P[2] - Q.P[2];
+    const double T = alpha * (X * X + Y * Y + Z * Z);
+    double F[MAX_LTOT + 1];
+    boys_runtime(L, T, boys, F);
+    double s = 1.0;
+    for (int m = 0; m <= L; ++m) { F[m] *= s; s *= -2.0 * alpha; }
+    double R[nherm(MAX_LTOT)];
+    hermite_R_rt(L, X, Y, Z, F, R);
+    double val = 0.0;
+    for (int t = 0; t < nbx; ++t)
+        for (int u = 0; u < nby; ++u)
+            for (int v = 0; v < nbz; ++v) {
+                const double eb3 = P.E[0][(la[0] * (lb[0] + 1) + lb[0]) * nbx + t]
+                                 * P.E[1][(la[1] * (lb[1] + 1) + lb[1]) * nby + u]
+                                 * P.E[2][(la[2] * (lb[2] + 1) + lb[2]) * nbz + v];
+                for (int tau = 0; tau < nkx; ++tau)
+                    for (int nu = 0; nu < nky; ++nu)
+                        for (int phi = 0; phi < nkz; ++phi) {
+                            const double sg = ((tau + nu + phi) & 1) ? -1.0 : 1.0;
+                            val += eb3 * sg
+                                 * Q.E[0][(lc[0] * (ld[0] + 1) + ld[0]) * nkx + tau]
+                                 * Q.E[1][(lc[1] * (ld[1] + 1) + ld[1]) * nky + nu]
+                                 * Q.E[2][(lc[2] * (ld[2] + 1) + ld[2]) * nkz + phi]
+                                 * R[hidx(t + tau, u + nu, v + phi)];
+                        }
+            }
+    return val * 34.98683665524972497 / (p * qq * sqrt(p + qq)) * P.w * Q.w;
+}
+
 __device__ double eri_bf_generic(const BFDesc& a, const BFDesc& b, const BFDesc& c, const BFDesc& d,
                                  const double* __restrict__ pexp, const double* __restrict__ pw,
                                  const double* __restrict__ boys)
 {
-    const int L = a.l + a.m + a.n + b.l + b.m + b.n + c.l + c.m + c.n + d.l + d.m + d.n;
-    const double A[3] = {a.x, a.y, a.z}, B[3] = {b.x, b.y, b.z}, C[3] = {c.x, c.y, c.z}, Dc[3] = {d.x, d.y, d.z};
-    const int la[3] = {a.l, a.m, a.n}, lb[3] = {b.l, b.m, b.n}, lc[3] = {c.l, c.m, c.n}, ld[3] = {d.l, d.m, d.n};
-    const int nbx = la[0] + lb[0] + 1, nby = la[1] + lb[1] + 1, nbz = la[2] + lb[2] + 1;
-    const int nkx = lc[0] + ld[0] + 1, nky = lc[1] + ld[1] + 1, nkz = lc[2] + ld[2] + 1;
     double total = 0.0;
+    PrimPairRT P, Q;
     for (int ia = 0; ia < a.np; ++ia)
         for (int ib = 0; ib < b.np; ++ib) {
-            const double ea = pexp[a.p0 + ia], eb = pexp[b.p0 + ib];
-            const double p = ea + eb;
-            double P[3], Eb[3][45], kab = 1.0;
-            for (int x = 0; x < 3; ++x) {
-                P[x] = (ea * A[x] + eb * B[x]) / p;
-                kab *= exp(-ea * eb / p * (A[x] - B[x]) * (A[x] - B[x]));
-                etab_rt(la[x], lb[x], 0.5 / p, P[x] - A[x], P[x] - B[x], Eb[x]);
-            }
-            const double wab = pw[a.p0 + ia] * pw[b.p0 + ib] * kab;
+            prim_pair_rt(a, b, ia, ib, pexp, pw, P);
             for (int ic = 0; ic < c.np; ++ic)
                 for (int id = 0; id < d.np; ++id) {
-                    const double ec = pexp[c.p0 + ic], ed = pexp[d.p0 + id];
-                    const double qq = ec + ed;
-                    double Q[3], Ek[3][45], kcd = 1.0;
-                    for (int x = 0; x < 3; ++x) {
-                        Q[x] = (ec * C[x] + ed * Dc[x]) / qq;
-                        kcd *= exp(-ec * ed / qq * (C[x] - Dc[x]) * (C[x] - Dc[x]));
-                        etab_rt(lc[x], ld[x], 0.5 / qq, Q[x] - C[x], Q[x] - Dc[x], Ek[x]);
-                    }
-                    const double wcd = pw[c.p0 + ic] * pw[d.p0 + id] * kcd;
-                    const double alpha = p * qq / (p + qq);
-                    const double X = P[0] - Q[0], Y = P[1] - Q[1], Z = P[2] - Q[2];
-                    const double T = alpha * (X * X + Y * Y + Z * Z);
-                    double F[MAX_LTOT + 1];
-                    boys_runtime(L, T, boys, F);
-                    double s = 1.0;
-                    for (int m = 0; m <= L; ++m) { F[m] *= s; s *= -2.0 * alpha; }
-                    double R[nherm(MAX_LTOT)];
-                    hermite_R_rt(L, X, Y, Z, F, R);
-                    double val = 0.0;
-                    for (int t = 0; t < nbx; ++t)
-                        for (int u = 0; u < nby; ++u)
-                            for (int v = 0; v < nbz; ++v) {
-                                const double eb3 = Eb[0][(la[0] * (lb[0] + 1) + lb[0]) * nbx + t]
-                                                 * Eb[1][(la[1] * (lb[1] + 1) + lb[1]) * nby + u]
-                                                 * Eb[2][(la[2] * (lb[2] + 1) + lb[2]) * nbz + v];
-                                for (int tau = 0; tau < nkx; ++tau)
-                                    for (int nu = 0; nu < nky; ++nu)
-                                        for (int phi = 0; phi < nkz; ++phi) {
-                                            const double sg = ((tau + nu + phi) & 1) ? -1.0 : 1.0;
-                                            val += eb3 * sg
-                                                 * Ek[0][(lc[0] * (ld[0] + 1) + ld[0]) * nkx + tau]
-                                                 * Ek[1][(lc[1] * (ld[1] + 1) + ld[1]) * nky + nu]
-                                                 * Ek[2][(lc[2] * (ld[2] + 1) + ld[2]) * nkz + phi]
-                                                 * R[hidx(t + tau, u + nu, v + phi)];
-                                        }
-                            }
-                    total += val * 34.98683665524972497 / (p * qq * sqrt(p + qq)) * wab * wcd;
+                    prim_pair_rt(c, d, ic, id, pexp, pw, Q);
+                    total += prim_quartet_rt(a, b, c, d, P, Q, boys);
                 }
         }
     return total;
+}
+
+// Schwarz diagonal (pq|pq), p >= q (mmd/molecule.py:93-99): one wavefront per
+// basis-function pair, its primitive quartets spread over the lanes and
+// summed by a wavefront reduction, so that heavily contracted core pairs
+// (up to 9^4 primitive quartets) do not serialise on one lane.
+__global__ void __launch_bounds__(256) eri_diag_wave_kernel(int N, const BFDesc* __restrict__ bfs,
+                                                            const double* __restrict__ pexp,
+                                                            const double* __restrict__ pw,
+                                                            const double* __restrict__ boys, double* out)
+{
+    const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    const int64_t npair = (int64_t)N * (N + 1) / 2;
+    if (w >= npair) return;   // uniform across the wavefront
+    int64_t r = (int64_t)((sqrt(8.0 * (double)w + 1.0) - 1.0) * 0.5);
+    while (r * (r + 1) / 2 > w) --r;
+    while ((r + 1) * (r + 2) / 2 <= w) ++r;
+    const int p = (int)r, q = (int)(w - r * (r + 1) / 2);
+    const BFDesc a = bfs[p], b = bfs[q];
+    const int nab = a.np * b.np;
+    double total = 0.0;
+    PrimPairRT P, Q;
+    for (int k = lane; k < nab * nab; k += 64) {
+        const int i = k / nab, j = k - i * nab;
+        prim_pair_rt(a, b, i / b.np, i % b.np, pexp, pw, P);
+        prim_pair_rt(a, b, j / b.np, j % b.np, pexp, pw, Q);
+        total += prim_quartet_rt(a, b, a, b, P, Q, boys);
+    }
+    for (int off = 32; off > 0; off >>= 1) total += __shfl_xor(total, off);
+    if (lane == 0) {
+        out[(size_t)p * N + q] = total;
+        out[(size_t)q * N + p] = total;
+    }
+}
+
+hipError_t launch_eri_diag(int N, const BFDesc* bfs, const double* pexp, const double* pw, const double* boys,
+                           double* out, hipStream_t st)
+{
+    const int64_t npair = (int64_t)N * (N + 1) / 2;
+    if (npair <= 0) return hipSuccess;
+    const int64_t grid = (npair + 3) / 4;   // four wavefronts per 256-thread block
+    hipLaunchKernelGGL(eri_diag_wave_kernel, dim3((unsigned)grid), dim3(256), 0, st, N, bfs, pexp, pw, boys, out);
+    return hipGetLastError();
 }
 
 __global__ void __launch_bounds__(256) eri_bf_list_kernel(int64_t nq, const int64_t* __restrict__ ijkl,
@@ -178,7 +236,11 @@ hipError_t launch_canonical_quartets(int mode, int N, const BFDesc* bfs, const d
 
 // ---------------------------------------------------------------------------
 // One-electron integrals (cython/onee.pyx:14-192, assembled as
-// mmd/molecule.py:253-276).  One lane per basis-function pair i >= j.
+// mmd/molecule.py:253-276).  One wavefront per basis-function pair i >= j;
+// its lanes take the (primitive pair, nucleus) combinations, so that the
+// nuclear attraction of heavily contracted pairs is spread over the lanes;
+// the overlap, kinetic, dipole and angular-momentum terms are added by the
+// lanes of the first nucleus.  A wavefront reduction sums the lanes.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) one_electron_kernel(int nbf, const BFDesc* __restrict__ bfs,
                                                            const double* __restrict__ pexp,
@@ -187,9 +249,10 @@ __global__ void __launch_bounds__(256) one_electron_kernel(int nbf, const BFDesc
                                                            double cz, const double* __restrict__ boys, double* S,
                                                            double* T, double* V, double* M, double* Lm)
 {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t idx = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
     const int64_t npair = (int64_t)nbf * (nbf + 1) / 2;
-    if (idx >= npair) return;
+    if (idx >= npair) return;   // uniform across the wavefront
     int i = (int)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
     while ((int64_t)i * (i + 1) / 2 > idx) --i;
     while ((int64_t)(i + 1) * (i + 2) / 2 <= idx) ++i;
@@ -198,25 +261,29 @@ __global__ void __launch_bounds__(256) one_electron_kernel(int nbf, const BFDesc
     const double A[3] = {a.x, a.y, a.z}, B[3] = {b.x, b.y, b.z}, Cc[3] = {cx, cy, cz};
     const int l1[3] = {a.l, a.m, a.n}, l2[3] = {b.l, b.m, b.n};
     double s = 0, t = 0, v = 0, mu[3] = {0, 0, 0}, lam[3] = {0, 0, 0};
-    for (int ia = 0; ia < a.np; ++ia)
-        for (int ib = 0; ib < b.np; ++ib) {
-            const double ea = pexp[a.p0 + ia], eb = pexp[b.p0 + ib];
-            const double p = ea + eb;
-            const double w = pw[a.p0 + ia] * pw[b.p0 + ib];
-            double P[3], kab = 1.0;
-            // E tables up to (l1+1, l2+2) per dimension: 4 x 5 x 8 entries.
-            double E[3][160];
-            int LI[3], LJ[3], NT[3];
-            for (int x = 0; x < 3; ++x) {
-                P[x] = (ea * A[x] + eb * B[x]) / p;
-                kab *= exp(-ea * eb / p * (A[x] - B[x]) * (A[x] - B[x]));
-                LI[x] = l1[x] + 1;
-                LJ[x] = l2[x] + 2;
-                NT[x] = LI[x] + LJ[x] + 1;
-                etab_rt(LI[x], LJ[x], 0.5 / p, P[x] - A[x], P[x] - B[x], E[x]);
-            }
+    const int nat = natom > 0 ? natom : 1;
+    const int nwork = a.np * b.np * nat;
+    for (int k = lane; k < nwork; k += 64) {
+        const int at = k % nat, ipp = k / nat;
+        const int ia = ipp / b.np, ib = ipp % b.np;
+        const double ea = pexp[a.p0 + ia], eb = pexp[b.p0 + ib];
+        const double p = ea + eb;
+        const double w = pw[a.p0 + ia] * pw[b.p0 + ib];
+        double P[3], kab = 1.0;
+        // E tables up to (l1+1, l2+2) per dimension: 4 x 5 x 8 entries.
+        double E[3][160];
+        int LI[3], LJ[3], NT[3];
+        for (int x = 0; x < 3; ++x) {
+            P[x] = (ea * A[x] + eb * B[x]) / p;
+            kab *= exp(-ea * eb / p * (A[x] - B[x]) * (A[x] - B[x]));
+            LI[x] = l1[x] + 1;
+            LJ[x] = l2[x] + 2;
+            NT[x] = LI[x] + LJ[x] + 1;
+            etab_rt(LI[x], LJ[x], 0.5 / p, P[x] - A[x], P[x] - B[x], E[x]);
+        }
 #define E1(x, ii, jj, tt) \
     (((jj) < 0 || (tt) > (ii) + (jj)) ? 0.0 : E[x][((ii) * (LJ[x] + 1) + (jj)) * NT[x] + (tt)])
+        if (at == 0) {
             const double pre = pow(3.141592653589793238462643383279 / p, 1.5) * kab * w;
             double S0[3], S1[3], D1[3], Tk[3], Dp[3];
             for (int x = 0; x < 3; ++x) {
@@ -239,27 +306,38 @@ __global__ void __launch_bounds__(256) one_electron_kernel(int nbf, const BFDesc
             lam[0] += -S0[0] * (S1[1] * D1[2] - S1[2] * D1[1]) * pre;
             lam[1] += -S0[1] * (S1[2] * D1[0] - S1[0] * D1[2]) * pre;
             lam[2] += -S0[2] * (S1[0] * D1[1] - S1[1] * D1[0]) * pre;
-            // nuclear attraction (cython/onee.pyx:175-192), summed over atoms
-            const int Lv = l1[0] + l2[0] + l1[1] + l2[1] + l1[2] + l2[2];
-            for (int at = 0; at < natom; ++at) {
-                const double Z = atoms[4 * at];
-                const double X = P[0] - atoms[4 * at + 1], Y = P[1] - atoms[4 * at + 2], W = P[2] - atoms[4 * at + 3];
-                double F[2 * MAX_L + 1];
-                boys_runtime(Lv, p * (X * X + Y * Y + W * W), boys, F);
-                double sc = 1.0;
-                for (int m = 0; m <= Lv; ++m) { F[m] *= sc; sc *= -2.0 * p; }
-                double R[nherm(2 * MAX_L)];
-                hermite_R_rt(Lv, X, Y, W, F, R);
-                double val = 0.0;
-                for (int tt = 0; tt <= l1[0] + l2[0]; ++tt)
-                    for (int uu = 0; uu <= l1[1] + l2[1]; ++uu)
-                        for (int vv = 0; vv <= l1[2] + l2[2]; ++vv)
-                            val += E1(0, l1[0], l2[0], tt) * E1(1, l1[1], l2[1], uu) * E1(2, l1[2], l2[2], vv)
-                                 * R[hidx(tt, uu, vv)];
-                v += -Z * val * 2.0 * 3.141592653589793238462643383279 / p * kab * w;
-            }
-#undef E1
         }
+        // nuclear attraction of nucleus `at` (cython/onee.pyx:175-192)
+        if (at < natom) {
+            const int Lv = l1[0] + l2[0] + l1[1] + l2[1] + l1[2] + l2[2];
+            const double Z = atoms[4 * at];
+            const double X = P[0] - atoms[4 * at + 1], Y = P[1] - atoms[4 * at + 2], W = P[2] - atoms[4 * at + 3];
+            double F[2 * MAX_L + 1];
+            boys_runtime(Lv, p * (X * X + Y * Y + W * W), boys, F);
+            double sc = 1.0;
+            for (int m = 0; m <= Lv; ++m) { F[m] *= sc; sc *= -2.0 * p; }
+            double R[nherm(2 * MAX_L)];
+            hermite_R_rt(Lv, X, Y, W, F, R);
+            double val = 0.0;
+            for (int tt = 0; tt <= l1[0] + l2[0]; ++tt)
+                for (int uu = 0; uu <= l1[1] + l2[1]; ++uu)
+                    for (int vv = 0; vv <= l1[2] + l2[2]; ++vv)
+                        val += E1(0, l1[0], l2[0], tt) * E1(1, l1[1], l2[1], uu) * E1(2, l1[2], l2[2], vv)
+                             * R[hidx(tt, uu, vv)];
+            v += -Z * val * 2.0 * 3.141592653589793238462643383279 / p * kab * w;
+        }
+#undef E1
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        s += __shfl_xor(s, off);
+        t += __shfl_xor(t, off);
+        v += __shfl_xor(v, off);
+        for (int x = 0; x < 3; ++x) {
+            mu[x] += __shfl_xor(mu[x], off);
+            lam[x] += __shfl_xor(lam[x], off);
+        }
+    }
+    if (lane != 0) return;
     const size_t n = (size_t)nbf;
     S[i * n + j] = S[j * n + i] = s;
     T[i * n + j] = T[j * n + i] = t;
@@ -277,7 +355,8 @@ hipError_t launch_one_electron(int nbf, const BFDesc* bfs, const double* pexp, c
 {
     const int64_t npair = (int64_t)nbf * (nbf + 1) / 2;
     if (npair <= 0) return hipSuccess;
-    hipLaunchKernelGGL(one_electron_kernel, dim3((unsigned)((npair + 255) / 256)), dim3(256), 0, st, nbf, bfs, pexp,
+    // four wavefronts (basis-function pairs) per 256-thread block
+    hipLaunchKernelGGL(one_electron_kernel, dim3((unsigned)((npair + 3) / 4)), dim3(256), 0, st, nbf, bfs, pexp,
                        pw, natom, atoms, cx, cy, cz, boys, S, T, V, M, Lm);
     return hipGetLastError();
 }

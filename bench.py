@@ -111,6 +111,102 @@ def cpu_baseline(geom, basis, n_builds, seconds):
             "fock_build_s": per_q * nquart}
 
 
+def bench_tier(args):
+    """(H2O)16 / cc-pVDZ (400 functions, 9 A cube, seed 0): direct RHF in
+    FP64 and with the FP32 tier (quartets with bound < 1e-4 in FP32)."""
+    import numpy as np
+    from mmd.molecule import Molecule
+    from mmd.workloads import water_cluster
+    geom = water_cluster(16, 9.0, seed=0)
+
+    def run(fp32_below):
+        mol = Molecule(geom, "cc-pvdz")
+        mol.native.set_precision(fp32_below)
+        tiers = []
+        orig = mol.native.fock_delta
+
+        def wrapped(*a, **k):
+            r = orig(*a, **k)
+            tiers.append(mol.native.fock_tiers())
+            return r
+        mol.native.fock_delta = wrapped
+        mol.RHF(doPrint=False, direct=True, conver=1e-8, acc2e=1e-12)
+        mol._tiers = tiers
+        return mol
+
+    for _ in range(args.warmup):
+        run(1e-4)
+    res = {}
+    for name, thr in (("fp64", 0.0), ("tier", 1e-4)):
+        t0 = time.perf_counter()
+        mols = [run(thr) for _ in range(args.steps)]
+        el = time.perf_counter() - t0
+        print("%s done" % name, file=sys.stderr, flush=True)
+        m = mols[-1]
+        nq = sum(a for a, b in m._tiers)
+        n32 = sum(b for a, b in m._tiers)
+        res[name] = {"scf_per_s": args.steps / el, "ms_per_scf": 1e3 * el / args.steps,
+                     "energy": float(m.energy.real), "iterations": int(m.iterations),
+                     "fock_build_ms": float(np.median(m.fock_ms[1:])), "fp32_quartet_fraction": n32 / max(1, nq)}
+    dE = abs(res["tier"]["energy"] - res["fp64"]["energy"])
+    N = Molecule(geom, "cc-pvdz").nbasis
+    return {
+        "metric": METRIC, "value": res["tier"]["scf_per_s"] if dE <= 1e-6 else 0.0, "unit": "SCF/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["tier"]["ms_per_scf"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
+        "config": {"workload": "(H2O)16 cc-pVDZ direct RHF, FP32 tier below 1e-4", "n_basis": N, "seed": 0,
+                   "box_A": 9.0, "min_OO_A": 2.7, "acc2e": 1e-12, "conver": 1e-8},
+        "delta_energy_vs_fp64": dE, "fp64": res["fp64"], "tier": res["tier"],
+    }
+
+
+def bench_batch(args):
+    """512 x (H2O)n, n uniform in 2..6, perturbed monomers (seed 1), 6-31G:
+    all direct FP64 RHF calls on one GPU through a warm pool of batch worker
+    processes (pool shape autotuned on a 64-molecule calibration batch); the
+    timed region is the 512-molecule batch."""
+    import numpy as np
+    from mmd.batch import BatchPool
+    from mmd.molecule import Molecule
+    from mmd.workloads import water_cluster
+    rng = np.random.default_rng(1)
+    ns = rng.integers(2, 7, size=512)
+    geoms = [water_cluster(int(n), 2.0 + 1.4 * int(n), seed=1000 + i, perturb=True) for i, n in enumerate(ns)]
+    # autotune the pool shape on a warm pool and a 64-molecule calibration
+    # batch; worker processes start before this process uses the GPU
+    rates = {}
+    for shape in ((4, 2), (4, 3), (4, 4), (5, 2)):
+        with BatchPool(*shape) as pool:
+            pool.run(geoms[:2 * shape[0] * shape[1]], "6-31g")        # warm-up: contexts, caches
+            t0 = time.perf_counter()
+            pool.run(geoms[:64], "6-31g")
+            rates[shape] = 64 / (time.perf_counter() - t0)
+        print("pool %dx%d: %.1f SCF/s" % (shape[0], shape[1], rates[shape]), file=sys.stderr, flush=True)
+    best = max(rates, key=rates.get)
+    with BatchPool(*best) as pool:
+        pool.run(geoms[:2 * best[0] * best[1]], "6-31g")
+        t0 = time.perf_counter()
+        res = pool.run(geoms, "6-31g")
+        el = time.perf_counter() - t0
+    # per-molecule energies against their own single-molecule runs (sample)
+    dmax = 0.0
+    for i in range(0, 512, 64):
+        m = Molecule(geoms[i], "6-31g")
+        m.RHF(doPrint=False, direct=True)
+        dmax = max(dmax, abs(m.energy.real - res[i]["energy"]))
+    return {
+        "metric": METRIC, "value": len(geoms) / el, "unit": "SCF/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": 1e3 * el, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "512 x (H2O)n n=2..6 6-31G direct RHF, batched", "seed": 1, "basis": "6-31g",
+                   "procs": best[0], "threads_per_proc": best[1],
+                   "autotune_scf_per_s": {"%dx%d" % k: v for k, v in rates.items()}},
+        "all_converged": all(r["converged"] for r in res),
+        "mean_iterations": float(np.mean([r["iterations"] for r in res])),
+        "max_abs_dE_vs_single_runs": dmax,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +215,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--workload", choices=("h2o8", "h2o16-tier", "batch512"), default="h2o8",
+                    help="h2o8: (H2O)8 cc-pVDZ FP64 (default); h2o16-tier: (H2O)16 cc-pVDZ FP64 vs FP32 tier; "
+                         "batch512: 512 small 6-31G clusters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -128,9 +227,15 @@ def main():
     import numpy as np
     from mmd.workloads import water_cluster
     from mmd import _native
+    if args.workload == "batch512":   # starts its worker processes before any GPU use here
+        print(json.dumps(bench_batch(args)))
+        return
     if _native.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.workload == "h2o16-tier":
+        print(json.dumps(bench_tier(args)))
+        return
 
     geom = water_cluster(8, 7.0, seed=0)
     basis = "cc-pvdz"
@@ -174,7 +279,7 @@ def main():
         achieved = f / (t * 1e-3) / 1e12
         out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                           "kernel": "eri_fock_class_kernel<%d,%d,%d,%d>" % dom,
+                           "kernel": "eri_fock_class_kernel<%d,%d,%d,%d,double>" % dom,
                            "kernel_avg_ms": t / n, "launches": n,
                            "all_classes_achieved": tot_f / (tot_t * 1e-3) / 1e12,
                            "all_classes_ms_per_build": tot_t / max(1, len(pm._profile_stats))}

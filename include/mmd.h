@@ -151,6 +151,43 @@ int mmd_fock_stats(mmd_ctx *ctx, int32_t *lclass, int64_t *nquart, int64_t *npri
 /* Device time (ms) of the last mmd_fock_delta call, from HIP events. */
 int mmd_last_fock_ms(mmd_ctx *ctx, double *ms);
 
+/*
+ * Precision tier of the symmetrised Fock build.  No counterpart in the
+ * reference (formPT is FP64 throughout, cython/fock.pyx:72-85): shell
+ * quartets whose screening bound q_ab q_cd max|dP| lies in [tol, fp32_below)
+ * have their integrals computed in FP32 (digestion into G stays FP64); the
+ * rest in FP64.  fp32_below = 0 (the default) keeps every quartet in FP64.
+ * mmd_fock_tiers reports the surviving shell quartets of the last
+ * symmetrised build and how many of them took the FP32 tier.
+ */
+int mmd_set_precision(mmd_ctx *ctx, double fp32_below);
+int mmd_fock_tiers(mmd_ctx *ctx, int64_t *nquart, int64_t *nquart_fp32);
+
+/*
+ * Multilayer perceptron of the learned initial density (no counterpart in
+ * the reference): tanh hidden layers, linear output layer, FP32 HIP kernels.
+ *   dims            : nlayer + 1 widths (input, hidden..., output)
+ *   params          : flat float32, per layer W (fan_in x fan_out, row
+ *                     major) then b (fan_out); mmd_mlp_nparam gives the size
+ *   forward         : Y (n x dims[nlayer]) = f(X (n x dims[0]))
+ *   loss_grad       : loss = mean over all n*dims[nlayer] outputs of
+ *                     (Y - Yt)^2 and its gradient with respect to params
+ *   train           : epochs of Adam (beta 0.9 / 0.999, eps 1e-8) on
+ *                     shuffled full mini-batches; epoch_loss (may be NULL)
+ *                     receives the mean batch loss of each epoch
+ * Host pointers; calls are synchronous.
+ */
+typedef struct mmd_mlp mmd_mlp;
+int mmd_mlp_create(mmd_mlp **out, int nlayer, const int32_t *dims, int device_id);
+void mmd_mlp_destroy(mmd_mlp *mlp);
+int64_t mmd_mlp_nparam(mmd_mlp *mlp);
+int mmd_mlp_set_params(mmd_mlp *mlp, const float *params);
+int mmd_mlp_get_params(mmd_mlp *mlp, float *params);
+int mmd_mlp_forward(mmd_mlp *mlp, int n, const float *X, float *Y);
+int mmd_mlp_loss_grad(mmd_mlp *mlp, int n, const float *X, const float *Yt, float *loss, float *grad);
+int mmd_mlp_train(mmd_mlp *mlp, int n, const float *X, const float *Yt, int epochs, int batch, float lr,
+                  uint64_t seed, float *epoch_loss);
+
 #ifdef __cplusplus
 }
 #endif

@@ -83,6 +83,9 @@ struct mmd_ctx {
     hipStream_t side[NSIDE] = {};
     hipEvent_t evfork = nullptr, evjoin[NSIDE] = {};
     double last_fock_ms = 0.0;
+    int64_t last_nsurv = 0, last_nsurv_fp32 = 0;   // shell quartets of the last symmetrised build
+    double fp32_below = 0.0;                        // FP32 tier: quartets with bound below this (0: off)
+    DevBuf<float> d_boysf;
 
     // basis, per function
     int nbf = 0;
@@ -359,6 +362,7 @@ ClassArgs base_args(mmd_ctx* c)
     A.pairs = c->d_pairs.p;
     A.pp = c->d_pp.p;
     A.boys = c->d_boys.p;
+    A.boysf = c->d_boysf.p;
     A.gcomp = c->d_gcomp.p;
     A.N = c->nbf;
     return A;
@@ -369,11 +373,12 @@ ClassArgs base_args(mmd_ctx* c)
 // not fit in LDS) are screened into survivor lists first.
 int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol)
 {
-    HIP_TRY(c->d_rowctr.reserve(NQCLASS));
-    HIP_TRY(c->d_nprim.reserve(2 * NQCLASS));
-    HIP_TRY(launch_fill_int(NQCLASS, c->d_rowctr.p, 0, c->st));
+    HIP_TRY(c->d_rowctr.reserve(2 * NQCLASS));
+    HIP_TRY(c->d_nprim.reserve(3 * NQCLASS));
+    HIP_TRY(launch_fill_int(2 * NQCLASS, c->d_rowctr.p, 0, c->st));
     HIP_TRY(launch_fill_int(NQCLASS, c->d_count.p, 0, c->st));
-    HIP_TRY(launch_fill_int(4 * NQCLASS, reinterpret_cast<int*>(c->d_nprim.p), 0, c->st));
+    HIP_TRY(launch_fill_int(6 * NQCLASS, reinterpret_cast<int*>(c->d_nprim.p), 0, c->st));
+    const bool tier32 = c->fp32_below > tol;
     bool row_kernel[NQCLASS];
     int maxpp[NQCLASS];
     for (int qc = 0; qc < NQCLASS; ++qc) {
@@ -382,7 +387,7 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol)
         class_of(qc, &pcb, &pck);
         maxpp[qc] = 1;
         for (int P = c->pc_off[pcb]; P < c->pc_off[pcb + 1]; ++P) maxpp[qc] = std::max(maxpp[qc], c->pairs[P].npp);
-        row_kernel[qc] = !fock_class_is_wave(la, lb, lc, ld) && fock_class_lds(qc, c->nbf, maxpp[qc]) <= c->max_lds;
+        row_kernel[qc] = !fock_class_is_wave(la, lb, lc, ld) && fock_class_lds(qc, 0, c->nbf, maxpp[qc]) <= c->max_lds;
     }
     // Classes run concurrently on the main and side streams (longest first,
     // each to the least loaded stream), so that the small classes fill the
@@ -447,7 +452,18 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol)
             const int64_t units = (int64_t)nb * ((nk + seg - 1) / seg);
             R.stat_nquart = c->d_nprim.p + NQCLASS + qc;
             R.stat_nprim = c->profiling ? c->d_nprim.p + qc : nullptr;
-            HIP_TRY(launch_fock_class(qc, A, R, (int)std::min<int64_t>(units, 2048), sq));
+            R.tier_lo = tier32 ? c->fp32_below : 0.0;
+            R.tier_hi = HUGE_VAL;
+            const int grid = (int)std::min<int64_t>(units, 2048);
+            HIP_TRY(launch_fock_class(qc, 0, A, R, grid, sq));
+            if (tier32) {
+                // FP32 tier: the same units, quartets with tol <= bound < fp32_below
+                R.tier_lo = 0.0;
+                R.tier_hi = c->fp32_below;
+                R.row_counter = c->d_rowctr.p + NQCLASS + qc;
+                R.stat_nquart = c->d_nprim.p + 2 * NQCLASS + qc;
+                HIP_TRY(launch_fock_class(qc, 1, A, R, grid, sq));
+            }
         } else {
             HIP_TRY(launch_screen_class(c->pc_off[pcb], nb, c->pc_off[pck], nk, pcb == pck ? 1 : 0, c->d_pairs.p,
                                         c->d_Dsh.p, (int)c->shells.size(), tol, 0, c->d_surv.p + c->qc_off[qc],
@@ -468,12 +484,14 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol)
     }
     if (c->profiling) {
         int h[NQCLASS];
-        unsigned long long hp[2 * NQCLASS];
+        unsigned long long hp[3 * NQCLASS];
         HIP_TRY(hipMemcpyAsync(h, c->d_count.p, sizeof(int) * NQCLASS, hipMemcpyDeviceToHost, c->st));
         HIP_TRY(hipMemcpyAsync(hp, c->d_nprim.p, sizeof(hp), hipMemcpyDeviceToHost, c->st));
         HIP_TRY(hipStreamSynchronize(c->st));
         for (int qc = 0; qc < NQCLASS; ++qc) {
-            c->st_nquart[qc] = c->qc_total[qc] ? (row_kernel[qc] ? (int64_t)hp[NQCLASS + qc] : h[qc]) : 0;
+            c->st_nquart[qc] = c->qc_total[qc]
+                                   ? (row_kernel[qc] ? (int64_t)(hp[NQCLASS + qc] + hp[2 * NQCLASS + qc]) : h[qc])
+                                   : 0;
             c->st_nprim[qc] = c->qc_total[qc] ? (int64_t)hp[qc] : 0;
             float ms = 0.f;
             if (c->qc_total[qc]) HIP_TRY(hipEventElapsedTime(&ms, c->evc[2 * qc], c->evc[2 * qc + 1]));
@@ -519,6 +537,10 @@ int set_schwarz(mmd_ctx* ctx, const std::vector<double>& h)
 }
 
 }  // namespace
+
+namespace mmd {
+void set_last_error(const char* msg) { g_last_error = msg; }
+}  // namespace mmd
 
 // ===========================================================================
 // C ABI
@@ -566,6 +588,9 @@ int mmd_ctx_create(mmd_ctx** out, int device_id)
     boys_build_table(tab);
     if (e == hipSuccess) e = c->d_boys.reserve(tab.size());
     if (e == hipSuccess) e = hipMemcpy(c->d_boys.p, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice);
+    std::vector<float> tabf(tab.begin(), tab.end());
+    if (e == hipSuccess) e = c->d_boysf.reserve(tabf.size());
+    if (e == hipSuccess) e = hipMemcpy(c->d_boysf.p, tabf.data(), sizeof(float) * tabf.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         mmd_ctx_destroy(c);
         return fail(MMD_ERR_DEVICE, "context setup: %s", hipGetErrorString(e));
@@ -802,15 +827,21 @@ int mmd_fock_delta(mmd_ctx* ctx, const double* dP_re, const double* dP_im, doubl
         }
     }
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->st));
-    if (nsurv && !raw) {
+    if (!raw) {
+        // survivors: per-class lists (wave classes) + FP64 and FP32 row-kernel counts
         int h[NQCLASS];
-        unsigned long long hq[NQCLASS];
+        unsigned long long hq[2 * NQCLASS];
         HIP_TRY(hipMemcpyAsync(h, ctx->d_count.p, sizeof(int) * NQCLASS, hipMemcpyDeviceToHost, ctx->st));
         HIP_TRY(hipMemcpyAsync(hq, ctx->d_nprim.p + NQCLASS, sizeof(hq), hipMemcpyDeviceToHost, ctx->st));
         HIP_TRY(hipStreamSynchronize(ctx->st));
-        int64_t s = 0;
-        for (int i = 0; i < NQCLASS; ++i) s += (int64_t)h[i] + (int64_t)hq[i];
-        *nsurv = s;
+        int64_t s = 0, s32 = 0;
+        for (int i = 0; i < NQCLASS; ++i) {
+            s += (int64_t)h[i] + (int64_t)hq[i] + (int64_t)hq[NQCLASS + i];
+            s32 += (int64_t)hq[NQCLASS + i];
+        }
+        ctx->last_nsurv = s;
+        ctx->last_nsurv_fp32 = s32;
+        if (nsurv) *nsurv = s;
     } else if (nsurv) {
         *nsurv = -1;
     }
@@ -893,6 +924,22 @@ int mmd_fock_stats(mmd_ctx* ctx, int32_t* lclass, int64_t* nquart, int64_t* npri
         if (ms) ms[qc] = ctx->st_ms[qc];
     }
     return NQCLASS;
+}
+
+int mmd_set_precision(mmd_ctx* ctx, double fp32_below)
+{
+    if (!ctx) return fail(MMD_ERR_ARG, "null context");
+    if (!(fp32_below >= 0.0) || fp32_below > 1.0) return fail(MMD_ERR_ARG, "fp32_below must be in [0, 1]");
+    ctx->fp32_below = fp32_below;
+    return MMD_OK;
+}
+
+int mmd_fock_tiers(mmd_ctx* ctx, int64_t* nquart, int64_t* nquart_fp32)
+{
+    if (!ctx) return fail(MMD_ERR_ARG, "null context");
+    if (nquart) *nquart = ctx->last_nsurv;
+    if (nquart_fp32) *nquart_fp32 = ctx->last_nsurv_fp32;
+    return MMD_OK;
 }
 
 int mmd_last_fock_ms(mmd_ctx* ctx, double* ms)

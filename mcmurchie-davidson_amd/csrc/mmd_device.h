@@ -98,50 +98,56 @@ constexpr int BOYS_TAYLOR = 7;        // terms k = 0..6
 constexpr int BOYS_MMAX = MAX_LTOT + BOYS_TAYLOR - 1;   // 14
 constexpr int BOYS_ROWS = BOYS_MMAX + 1;
 
-// Rows needed by a class of total angular momentum L: m = L .. L+6.
-template <int L>
-__device__ inline void boys_stage_lds(double* lds, const double* __restrict__ tab)
+__device__ __forceinline__ double dev_rsqrt(double x) { return rsqrt(x); }
+__device__ __forceinline__ float dev_rsqrt(float x) { return rsqrtf(x); }
+__device__ __forceinline__ double dev_exp(double x) { return exp(x); }
+__device__ __forceinline__ float dev_exp(float x) { return expf(x); }
+
+// Rows needed by a class of total angular momentum L: m = L .. L+6.  The
+// table is double (FP64 kernels) or float (FP32-tier kernels).
+template <int L, typename R = double>
+__device__ inline void boys_stage_lds(R* lds, const R* __restrict__ tab)
 {
-    const double* src = tab + (size_t)L * BOYS_NT;
+    const R* src = tab + (size_t)L * BOYS_NT;
     for (int i = threadIdx.x; i < BOYS_TAYLOR * BOYS_NT; i += blockDim.x) lds[i] = src[i];
 }
 
 // F[0..L] scaled by pref * (-2 alpha)^m, i.e. the seeds of the R recursion.
-template <int L>
-__device__ inline void boys_seeds(double T, double alpha, double pref, const double* lds, double* F)
+template <int L, typename R = double>
+__device__ inline void boys_seeds(R T, R alpha, R pref, const R* lds, R* F)
 {
-    double Fm[L + 1];
-    if (T < BOYS_TMAX) {
-        const int i = (int)(T * BOYS_INV_DT + 0.5);
-        const double dT = (double)i * BOYS_DT - T;   // expansion in (-dT)
-        const double* r = lds + i;
+    R Fm[L + 1];
+    if (T < R(BOYS_TMAX)) {
+        const int i = (int)(T * R(BOYS_INV_DT) + R(0.5));
+        const R dT = (R)i * R(BOYS_DT) - T;   // expansion in (-dT)
+        const R* r = lds + i;
         // sum_k F_{L+k}(T_i) (T_i - T)^k / k!   (Horner)
-        double f = r[6 * BOYS_NT];
-        f = r[5 * BOYS_NT] + dT * f * (1.0 / 6.0);
-        f = r[4 * BOYS_NT] + dT * f * (1.0 / 5.0);
-        f = r[3 * BOYS_NT] + dT * f * (1.0 / 4.0);
-        f = r[2 * BOYS_NT] + dT * f * (1.0 / 3.0);
-        f = r[1 * BOYS_NT] + dT * f * (1.0 / 2.0);
+        R f = r[6 * BOYS_NT];
+        f = r[5 * BOYS_NT] + dT * f * R(1.0 / 6.0);
+        f = r[4 * BOYS_NT] + dT * f * R(1.0 / 5.0);
+        f = r[3 * BOYS_NT] + dT * f * R(1.0 / 4.0);
+        f = r[2 * BOYS_NT] + dT * f * R(1.0 / 3.0);
+        f = r[1 * BOYS_NT] + dT * f * R(1.0 / 2.0);
         f = r[0] + dT * f;
         Fm[L] = f;
         if (L > 0) {
-            const double e = exp(-T);
-            const double t2 = 2.0 * T;
+            const R e = dev_exp(-T);
+            const R t2 = R(2.0) * T;
 #pragma unroll
-            for (int m = L - 1; m >= 0; --m) Fm[m] = (t2 * Fm[m + 1] + e) * (1.0 / (2.0 * m + 1.0));
+            for (int m = L - 1; m >= 0; --m) Fm[m] = (t2 * Fm[m + 1] + e) * R(1.0 / (2.0 * m + 1.0));
         }
     } else {
-        const double rT = rsqrt(T);
-        const double inv2T = 0.5 * rT * rT;
-        Fm[0] = 0.886226925452758013649083741671 * rT;
+        const R rT = dev_rsqrt(T);
+        const R inv2T = R(0.5) * rT * rT;
+        Fm[0] = R(0.886226925452758013649083741671) * rT;
         if (L > 0) {
-            const double e = exp(-T);
+            const R e = dev_exp(-T);
 #pragma unroll
-            for (int m = 0; m < L; ++m) Fm[m + 1] = ((2.0 * m + 1.0) * Fm[m] - e) * inv2T;
+            for (int m = 0; m < L; ++m) Fm[m + 1] = (R(2.0 * m + 1.0) * Fm[m] - e) * inv2T;
         }
     }
-    const double m2a = -2.0 * alpha;
-    double s = pref;
+    const R m2a = R(-2.0) * alpha;
+    R s = pref;
 #pragma unroll
     for (int m = 0; m <= L; ++m) {
         F[m] = s * Fm[m];
@@ -155,8 +161,8 @@ __device__ inline void boys_seeds(double T, double alpha, double pref, const dou
 // cython/util.pxi:33-50, made iterative: level n from L down to 0, degree from
 // L-n down to 1, reusing the level n+1 values in the same registers).
 // ---------------------------------------------------------------------------
-template <int L>
-__device__ inline void hermite_R(double X, double Y, double Z, const double* F, double* R)
+template <int L, typename Real = double>
+__device__ inline void hermite_R(Real X, Real Y, Real Z, const Real* F, Real* R)
 {
     R[0] = F[L];
     sfor<L>([&](auto in) {
@@ -167,16 +173,16 @@ __device__ inline void hermite_R(double X, double Y, double Z, const double* F, 
                 constexpr int k = nherm(d - 1) + decltype(ik)::value;
                 constexpr Herm3 h = hdecode(k);
                 constexpr int t = h.t, u = h.u, v = h.v;
-                double val;
+                Real val;
                 if constexpr (t > 0) {
                     val = X * R[hidx(t - 1, u, v)];
-                    if constexpr (t > 1) val = fma((double)(t - 1), R[hidx(t - 2, u, v)], val);
+                    if constexpr (t > 1) val = fma((Real)(t - 1), R[hidx(t - 2, u, v)], val);
                 } else if constexpr (u > 0) {
                     val = Y * R[hidx(t, u - 1, v)];
-                    if constexpr (u > 1) val = fma((double)(u - 1), R[hidx(t, u - 2, v)], val);
+                    if constexpr (u > 1) val = fma((Real)(u - 1), R[hidx(t, u - 2, v)], val);
                 } else {
                     val = Z * R[hidx(t, u, v - 1)];
-                    if constexpr (v > 1) val = fma((double)(v - 1), R[hidx(t, u, v - 2)], val);
+                    if constexpr (v > 1) val = fma((Real)(v - 1), R[hidx(t, u, v - 2)], val);
                 }
                 R[k] = val;
             });
@@ -194,21 +200,21 @@ __device__ inline void hermite_R(double X, double Y, double Z, const double* F, 
 // Storage E[(i*(LJ+1)+j)*(LI+LJ+1)+t].  SIGN = -1 multiplies odd t by -1
 // (the (-1)^{tau+nu+phi} factor of the ket, cython/twoe.pyx:89).
 // ---------------------------------------------------------------------------
-template <int LI, int LJ, int SIGN>
-__device__ inline void hermite_E(double inv2p, double PA, double PB, double* E)
+template <int LI, int LJ, int SIGN, typename Real = double>
+__device__ inline void hermite_E(Real inv2p, Real PA, Real PB, Real* E)
 {
     constexpr int NT = LI + LJ + 1;
     constexpr auto EI = [](int i, int j, int t) { return (i * (LJ + 1) + j) * NT + t; };
-    sfor<(LI + 1) * (LJ + 1) * NT>([&](auto k) { E[k] = 0.0; });
-    E[0] = 1.0;
+    sfor<(LI + 1) * (LJ + 1) * NT>([&](auto k) { E[k] = Real(0); });
+    E[0] = Real(1);
     sfor<LI>([&](auto ii) {
         constexpr int i = decltype(ii)::value;
         sfor<i + 2>([&](auto it) {
             constexpr int t = decltype(it)::value;
-            double v = 0.0;
+            Real v = Real(0);
             if constexpr (t > 0) v = fma(inv2p, E[EI(i, 0, t - 1)], v);
             if constexpr (t <= i) v = fma(PA, E[EI(i, 0, t)], v);
-            if constexpr (t + 1 <= i) v = fma((double)(t + 1), E[EI(i, 0, t + 1)], v);
+            if constexpr (t + 1 <= i) v = fma((Real)(t + 1), E[EI(i, 0, t + 1)], v);
             E[EI(i + 1, 0, t)] = v;
         });
     });
@@ -218,10 +224,10 @@ __device__ inline void hermite_E(double inv2p, double PA, double PB, double* E)
             constexpr int j = decltype(ij)::value;
             sfor<i + j + 2>([&](auto it) {
                 constexpr int t = decltype(it)::value;
-                double v = 0.0;
+                Real v = Real(0);
                 if constexpr (t > 0) v = fma(inv2p, E[EI(i, j, t - 1)], v);
                 if constexpr (t <= i + j) v = fma(PB, E[EI(i, j, t)], v);
-                if constexpr (t + 1 <= i + j) v = fma((double)(t + 1), E[EI(i, j, t + 1)], v);
+                if constexpr (t + 1 <= i + j) v = fma((Real)(t + 1), E[EI(i, j, t + 1)], v);
                 E[EI(i, j + 1, t)] = v;
             });
         });

@@ -1,29 +1,34 @@
 // One shell-quartet class of the Fock-build kernel per translation unit; the
-// build sets QC_LA, QC_LB, QC_LC, QC_LD.  Splitting the 21 fully unrolled
-// classes over separate objects lets them compile in parallel.
+// build sets QC_LA, QC_LB, QC_LC, QC_LD, and QC_F32 for the FP32-tier
+// instantiation.  Splitting the classes over separate objects lets them
+// compile in parallel.
 #include "mmd_eri_impl.h"
 
-#define MMD_PASTE5(p, a, b, c, d) p##a##b##c##d
-#define MMD_NAME(p, a, b, c, d) MMD_PASTE5(p, a, b, c, d)
+#define MMD_PASTE6(p, a, b, c, d, s) p##a##b##c##d##s
+#define MMD_NAME(p, a, b, c, d, s) MMD_PASTE6(p, a, b, c, d, s)
+
+#ifdef QC_F32
+#define QC_T float
+#define QC_SUFFIX _f
+#else
+#define QC_T double
+#define QC_SUFFIX _d
+#endif
 
 namespace mmd {
 
-hipError_t MMD_NAME(fock_class_launch_, QC_LA, QC_LB, QC_LC, QC_LD)(const ClassArgs& A, const RowArgs& R, int grid,
-                                                                    hipStream_t st)
+hipError_t MMD_NAME(fock_class_launch_, QC_LA, QC_LB, QC_LC, QC_LD, QC_SUFFIX)(const ClassArgs& A, const RowArgs& R,
+                                                                               int grid, hipStream_t st)
 {
-    const size_t lds = fock_class_lds_bytes<QC_LA, QC_LB, QC_LC, QC_LD>(A.N, R.maxpp);
-    hipLaunchKernelGGL((eri_fock_class_kernel<QC_LA, QC_LB, QC_LC, QC_LD>), dim3(grid), dim3(256), lds, st, A, R);
+    const size_t lds = fock_class_lds_bytes<QC_LA, QC_LB, QC_LC, QC_LD, QC_T>(A.N, R.maxpp);
+    hipLaunchKernelGGL((eri_fock_class_kernel<QC_LA, QC_LB, QC_LC, QC_LD, QC_T>), dim3(grid), dim3(CLASS_BLOCK), lds,
+                       st, A, R);
     return hipGetLastError();
 }
 
-size_t MMD_NAME(fock_class_lds_, QC_LA, QC_LB, QC_LC, QC_LD)(int N, int maxpp)
+size_t MMD_NAME(fock_class_lds_, QC_LA, QC_LB, QC_LC, QC_LD, QC_SUFFIX)(int N, int maxpp)
 {
-    return fock_class_lds_bytes<QC_LA, QC_LB, QC_LC, QC_LD>(N, maxpp);
-}
-
-int MMD_NAME(fock_class_nchunk_, QC_LA, QC_LB, QC_LC, QC_LD)()
-{
-    return ClassShape<QC_LA, QC_LB, QC_LC, QC_LD>::NCHUNK;
+    return fock_class_lds_bytes<QC_LA, QC_LB, QC_LC, QC_LD, QC_T>(N, maxpp);
 }
 
 }  // namespace mmd

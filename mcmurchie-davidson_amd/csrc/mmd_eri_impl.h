@@ -60,6 +60,13 @@ struct ClassShape {
     static constexpr int NTB = LA + LB + 1, NTK = LC + LD + 1;
     static constexpr int EB = (LA + 1) * (LB + 1) * NTB;   // 1-D bra E table
     static constexpr int BPS = 8 + 3 * EB;                 // staged bra primitive stride (doubles)
+    // Classes with few integrals per quartet (the heavily contracted low-L
+    // ones) add the primitive-slice partials of a batch into an LDS buffer
+    // and digest each quartet once; QB is their batch size (buffer <= 16 KB).
+    static constexpr int NABCD = NAB * NCD;
+    static constexpr bool IBUF = NABCD <= 32;
+    static constexpr int QB = IBUF ? (2048 / NABCD >= 256 ? 256 : (2048 / NABCD >= 128 ? 128 : 64)) : 256;
+    static constexpr int IBUF_DOUBLES = IBUF ? QB * NABCD : 0;
 };
 
 __device__ __forceinline__ double dmax6(double a, double b, double c, double d, double e, double f)
@@ -69,25 +76,25 @@ __device__ __forceinline__ double dmax6(double a, double b, double c, double d, 
 
 // Stage the bra pair's primitive pairs in LDS: p, 2 pi^{5/2} K / p, P and the
 // three 1-D Hermite tables E^{ab}_t (cython/util.pxi:13-28).
-template <int LA, int LB, int LC, int LD>
-__device__ __forceinline__ void stage_bra(const ClassArgs& A, const ShellPair& bra, double* bp)
+template <int LA, int LB, int LC, int LD, typename T>
+__device__ __forceinline__ void stage_bra(const ClassArgs& A, const ShellPair& bra, T* bp)
 {
     using S = ClassShape<LA, LB, LC, LD>;
     for (int idx = threadIdx.x; idx < 3 * bra.npp; idx += blockDim.x) {
         const int ip = idx / 3, x = idx - 3 * ip;
         const PrimPair* P = A.pp + bra.pp0 + ip;
-        double* b = bp + ip * S::BPS;
+        T* b = bp + ip * S::BPS;
         if (x == 0) {
-            b[0] = P->p;
-            b[1] = TWO_PI_2_5 * P->K * P->invp;
-            b[2] = P->Px;
-            b[3] = P->Py;
-            b[4] = P->Pz;
+            b[0] = (T)P->p;
+            b[1] = (T)(TWO_PI_2_5 * P->K * P->invp);
+            b[2] = (T)P->Px;
+            b[3] = (T)P->Py;
+            b[4] = (T)P->Pz;
         }
         double E[S::EB];
         hermite_E<LA, LB, 1>(P->inv2p, x == 0 ? P->PAx : (x == 1 ? P->PAy : P->PAz),
                              x == 0 ? P->PBx : (x == 1 ? P->PBy : P->PBz), E);
-        sfor<S::EB>([&](auto k) { b[8 + x * S::EB + k] = E[k]; });
+        sfor<S::EB>([&](auto k) { b[8 + x * S::EB + k] = (T)E[k]; });
     }
 }
 
@@ -100,44 +107,41 @@ __device__ __forceinline__ void stage_bra(const ClassArgs& A, const ShellPair& b
 // R_tuv, and the bra contraction accumulated in the ket's Hermite space,
 //   V_{tau nu phi}(ab) += sum_tuv E^{ab}_tuv R_{t+tau,u+nu,v+phi};
 // per ket primitive: (ab|cd) += sum (-1)^{tau+nu+phi} E^{cd}_{tau nu phi} V(ab).
-// rowA / rowB: rows of G for the bra shells' functions (row x of shell a at
-// rowA + x*N) accumulated in LDS; the ket-side Coulomb block goes to G.
+// The integrals of the item (its bra components x all ket components) are
+// returned in acc; quartet_digest adds them to the Fock matrix.
 // ---------------------------------------------------------------------------
-template <int LA, int LB, int LC, int LD, int A0, int A1>
-__device__ __forceinline__ void quartet_item(const ClassArgs& A, const double* lds, const double* bp, int2 qd,
-                                             const ShellPair& bra, const ShellPair& ket, int ip0, int ip1, int jq0,
-                                             int jq1, double* rowA, double* rowB)
+template <int LA, int LB, int LC, int LD, int A0, int A1, typename T>
+__device__ __forceinline__ void quartet_compute(const ClassArgs& A, const T* lds, const T* bp,
+                                                const ShellPair& ket, int ip0, int ip1, int jq0, int jq1,
+                                                T* acc)
 {
     using S = ClassShape<LA, LB, LC, LD>;
     constexpr int NCH = A1 - A0;
     constexpr int NCD = S::NCD;
     constexpr int L = S::L;
     constexpr int HCD = S::HCD;
-    auto AI = [](int ab, int cd) { return (ab - A0) * NCD + cd; };
-
-    double acc[NCH * NCD];
-    sfor<NCH * NCD>([&](auto k) { acc[k] = 0.0; });
+    sfor<NCH * NCD>([&](auto k) { acc[k] = T(0); });
 
     for (int jq = jq0; jq < jq1; ++jq) {
         const PrimPair* Q = A.pp + ket.pp0 + jq;
-        const double Qp = Q->p, QK = Q->K * Q->invp, Qx = Q->Px, Qy = Q->Py, Qz = Q->Pz;
-        double V[HCD * NCH];
-        sfor<HCD * NCH>([&](auto k) { V[k] = 0.0; });
+        const T Qp = (T)Q->p, QK = (T)(Q->K * Q->invp), Qx = (T)Q->Px, Qy = (T)Q->Py, Qz = (T)Q->Pz;
+        T V[HCD * NCH];
+        sfor<HCD * NCH>([&](auto k) { V[k] = T(0); });
         for (int ip = ip0; ip < ip1; ++ip) {
-            const double* b = bp + ip * S::BPS;
-            const double Pp = b[0];
-            const double rs = rsqrt(Pp + Qp);
-            const double alpha = Pp * Qp * rs * rs;
-            const double X = b[2] - Qx, Y = b[3] - Qy, Z = b[4] - Qz;
-            const double T = alpha * (X * X + Y * Y + Z * Z);
-            const double pref = b[1] * QK * rs;   // 2 pi^{5/2} / (p q sqrt(p+q)) K_ab K_cd
-            double F[L + 1];
-            boys_seeds<L>(T, alpha, pref, lds, F);
-            double R[nherm(L)];
-            hermite_R<L>(X, Y, Z, F, R);
-            const double* Ebx = b + 8;
-            const double* Eby = b + 8 + S::EB;
-            const double* Ebz = b + 8 + 2 * S::EB;
+            const T* b = bp + ip * S::BPS;
+            const T Pp = b[0];
+            const T rs = dev_rsqrt(Pp + Qp);
+            const T alpha = Pp * Qp * rs * rs;
+            const T X = b[2] - Qx, Y = b[3] - Qy, Z = b[4] - Qz;
+            const T Targ = alpha * (X * X + Y * Y + Z * Z);
+            const T pref = b[1] * QK * rs;   // 2 pi^{5/2} / (p q sqrt(p+q)) K_ab K_cd
+            T F[L + 1];
+            boys_seeds<L, T>(Targ, alpha, pref, lds, F);
+            T R[nherm(L)];
+            hermite_R<L, T>(X, Y, Z, F, R);
+            const T* Ebx = b + 8;
+            const T* Eby = b + 8 + S::EB;
+            const T* Ebz = b + 8 + 2 * S::EB;
             sfor<NCH>([&](auto ii) {
                 constexpr int ab = A0 + decltype(ii)::value;
                 constexpr int a = ab / S::NB, bb = ab % S::NB;
@@ -148,12 +152,12 @@ __device__ __forceinline__ void quartet_item(const ClassArgs& A, const double* l
                 constexpr int iz = (az * (LB + 1) + bz) * S::NTB;
                 sfor<HCD>([&](auto ik) {
                     constexpr Herm3 h = hdecode(decltype(ik)::value);
-                    double v = V[decltype(ik)::value * NCH + decltype(ii)::value];
+                    T v = V[decltype(ik)::value * NCH + decltype(ii)::value];
                     sfor<ax + bx + 1>([&](auto it) {
                         constexpr int t = decltype(it)::value;
                         sfor<ay + by + 1>([&](auto iu) {
                             constexpr int u = decltype(iu)::value;
-                            const double exy = Ebx[ix + t] * Eby[iy + u];
+                            const T exy = Ebx[ix + t] * Eby[iy + u];
                             sfor<az + bz + 1>([&](auto iw) {
                                 constexpr int w = decltype(iw)::value;
                                 v = fma(exy * Ebz[iz + w], R[hidx(t + h.t, u + h.u, w + h.v)], v);
@@ -165,15 +169,15 @@ __device__ __forceinline__ void quartet_item(const ClassArgs& A, const double* l
             });
         }
         // ket expansion, once per ket primitive pair
-        double Ekx[(LC + 1) * (LD + 1) * S::NTK], Eky[(LC + 1) * (LD + 1) * S::NTK],
+        T Ekx[(LC + 1) * (LD + 1) * S::NTK], Eky[(LC + 1) * (LD + 1) * S::NTK],
             Ekz[(LC + 1) * (LD + 1) * S::NTK];
         if constexpr (LC + LD > 0) {
-            const double qinv2 = Q->inv2p;
-            hermite_E<LC, LD, -1>(qinv2, Q->PAx, Q->PBx, Ekx);
-            hermite_E<LC, LD, -1>(qinv2, Q->PAy, Q->PBy, Eky);
-            hermite_E<LC, LD, -1>(qinv2, Q->PAz, Q->PBz, Ekz);
+            const T qinv2 = (T)Q->inv2p;
+            hermite_E<LC, LD, -1, T>(qinv2, (T)Q->PAx, (T)Q->PBx, Ekx);
+            hermite_E<LC, LD, -1, T>(qinv2, (T)Q->PAy, (T)Q->PBy, Eky);
+            hermite_E<LC, LD, -1, T>(qinv2, (T)Q->PAz, (T)Q->PBz, Ekz);
         } else {
-            Ekx[0] = Eky[0] = Ekz[0] = 1.0;
+            Ekx[0] = Eky[0] = Ekz[0] = T(1);
         }
         sfor<NCD>([&](auto icd) {
             constexpr int cd = decltype(icd)::value;
@@ -184,12 +188,12 @@ __device__ __forceinline__ void quartet_item(const ClassArgs& A, const double* l
             constexpr int ky = (cy * (LD + 1) + dy) * S::NTK;
             constexpr int kz = (cz * (LD + 1) + dz) * S::NTK;
             sfor<NCH>([&](auto ii) {
-                double sacc = acc[decltype(ii)::value * NCD + cd];
+                T sacc = acc[decltype(ii)::value * NCD + cd];
                 sfor<cx + dx + 1>([&](auto itau) {
                     constexpr int tau = decltype(itau)::value;
                     sfor<cy + dy + 1>([&](auto inu) {
                         constexpr int nu = decltype(inu)::value;
-                        const double exy = Ekx[kx + tau] * Eky[ky + nu];
+                        const T exy = Ekx[kx + tau] * Eky[ky + nu];
                         sfor<cz + dz + 1>([&](auto iphi) {
                             constexpr int phi = decltype(iphi)::value;
                             sacc = fma(exy * Ekz[kz + phi], V[hidx(tau, nu, phi) * NCH + decltype(ii)::value], sacc);
@@ -200,7 +204,19 @@ __device__ __forceinline__ void quartet_item(const ClassArgs& A, const double* l
             });
         });
     }
+}
 
+// Digest the integrals acc of a quartet (bra components A0..A1-1) into G.
+// rowA / rowB: rows of G for the bra shells' functions (row x of shell a at
+// rowA + x*N) accumulated in LDS; the ket-side Coulomb block goes to G.
+template <int LA, int LB, int LC, int LD, int A0, int A1>
+__device__ __forceinline__ void quartet_digest(const ClassArgs& A, int2 qd, const ShellPair& bra,
+                                               const ShellPair& ket, double* acc, double* rowA, double* rowB)
+{
+    using S = ClassShape<LA, LB, LC, LD>;
+    constexpr int NCH = A1 - A0;
+    constexpr int NCD = S::NCD;
+    auto AI = [](int ab, int cd) { return (ab - A0) * NCD + cd; };
     // -----------------------------------------------------------------------
     // Epilogue: component scales, shell-level degeneracy, basis-function-level
     // Schwarz x |dP| test (cython/fock.pyx:46-57), then Coulomb and exchange.
@@ -353,32 +369,46 @@ __device__ __forceinline__ void static_switch(int i, F&& f)
 // ---------------------------------------------------------------------------
 // Class kernel.  A work unit is one bra shell pair ("row") and a segment of
 // its ket pairs; workgroups take units from a device counter.  The unit's
-// ket pairs are screened 256 at a time; survivors are appended to an LDS list
-// that is digested whenever it holds a full batch, as wave items of (64
-// quartets, bra-component chunk, bra primitive slice, ket primitive slice),
-// so that lanes stay busy and heavily contracted quartets are spread over
-// several lanes.  Coulomb
-// G_ab and the four exchange blocks, all in the rows of the bra shells, are
-// accumulated in LDS (ds_add_f64) and their nonzero entries added to G once
-// per unit; only the
+// ket pairs are screened a workgroup at a time; survivors are appended to an
+// LDS list that is digested whenever it holds a full batch, as wave items of
+// (64 quartets, bra-component chunk, bra primitive slice, ket primitive
+// slice), so that lanes stay busy and heavily contracted quartets are spread
+// over several lanes.  Classes with an integral buffer (IBUF) add each
+// item's partial integrals into LDS and digest every quartet once, after the
+// batch; the others digest every item.  Coulomb G_ab and the four exchange
+// blocks, all in the rows of the bra shells, are accumulated in LDS
+// (ds_add_f64) and their nonzero entries added to G once per unit; only the
 // ket-side Coulomb block is added to G directly.  (The host sends a class
 // whose bra rows do not fit in LDS to the wavefront-per-quartet kernel.)
 // ---------------------------------------------------------------------------
-constexpr int QBATCH = 256;                // survivors digested per batch
-constexpr int QLIST = QBATCH + 256;        // batch + one screening step
+constexpr int CLASS_BLOCK = 256;           // threads per workgroup
 constexpr int PSLICE = 8;                  // primitive pairs per slice (bra and ket)
 
-template <int LA, int LB, int LC, int LD>
-__global__ void __launch_bounds__(256) eri_fock_class_kernel(ClassArgs A, RowArgs R)
+template <int QB>
+struct ClassList {
+    static constexpr int QLIST = QB + CLASS_BLOCK;   // batch + one screening step
+};
+
+// T is the arithmetic type of the integrals (double, or float for the
+// FP32 tier); screening, digestion and all accumulation into G are FP64.
+template <int LA, int LB, int LC, int LD, typename T>
+__global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A, RowArgs R)
 {
     using S = ClassShape<LA, LB, LC, LD>;
+    constexpr int QB = S::QB;
     extern __shared__ double dyn[];
-    double* lds_boys = dyn;
-    double* rows = dyn + BOYS_TAYLOR * BOYS_NT;
-    double* bp = rows + (S::NA + S::NB) * A.N;
+    double* rows = dyn;
+    double* ibuf = rows + (S::NA + S::NB) * A.N;
+    T* lds_boys = reinterpret_cast<T*>(ibuf + S::IBUF_DOUBLES);
+    T* bp = lds_boys + BOYS_TAYLOR * BOYS_NT;
     int* qlist = reinterpret_cast<int*>(bp + (size_t)R.maxpp * S::BPS);
-    int* misc = qlist + QLIST;
-    boys_stage_lds<S::L>(lds_boys, A.boys);
+    int* misc = qlist + ClassList<QB>::QLIST;
+    if constexpr (std::is_same<T, float>::value)
+        boys_stage_lds<S::L, float>(lds_boys, A.boysf);
+    else
+        boys_stage_lds<S::L, double>(lds_boys, A.boys);
+    if constexpr (S::IBUF)
+        for (int i = threadIdx.x; i < S::IBUF_DOUBLES; i += blockDim.x) ibuf[i] = 0.0;
     const int N = A.N;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int nwv = (int)(blockDim.x >> 6);
@@ -399,8 +429,10 @@ __global__ void __launch_bounds__(256) eri_fock_class_kernel(ClassArgs A, RowArg
         const int segn = min(R.seg, nk - k0);
         const ShellPair bra = A.pairs[R.boff + P];
         const int nrows = bra.same ? S::NA : S::NA + S::NB;
+        double* rowA = rows;
+        double* rowB = rows + (bra.same ? 0 : S::NA * N);
         for (int i = threadIdx.x; i < nrows * N; i += blockDim.x) rows[i] = 0.0;
-        stage_bra<LA, LB, LC, LD>(A, bra, bp);
+        stage_bra<LA, LB, LC, LD, T>(A, bra, bp);
         if (threadIdx.x == 0) {
             misc[1] = 0;   // survivors in the list
             misc[2] = 0;   // largest ket primitive-pair count in the list
@@ -419,7 +451,8 @@ __global__ void __launch_bounds__(256) eri_fock_class_kernel(ClassArgs A, RowArg
                     const double dm = fmax(fmax(fmax(4.0 * Dsh[bra.sa * ns + bra.sb], 4.0 * Dsh[ket.sa * ns + ket.sb]),
                                                 fmax(Dsh[bra.sa * ns + ket.sa], Dsh[bra.sa * ns + ket.sb])),
                                            fmax(Dsh[bra.sb * ns + ket.sa], Dsh[bra.sb * ns + ket.sb]));
-                    keep = bra.q * ket.q * dm >= A.tol;
+                    const double bound = bra.q * ket.q * dm;
+                    keep = bound >= A.tol && bound >= R.tier_lo && bound < R.tier_hi;
                     npk = ket.npp;
                 }
                 const unsigned long long m = __ballot(keep);
@@ -432,10 +465,11 @@ __global__ void __launch_bounds__(256) eri_fock_class_kernel(ClassArgs A, RowArg
                 }
             }
             __syncthreads();
-            const int cnt = misc[1];
+            const int cnt0 = misc[1];
             const bool last = t0 + (int)blockDim.x >= segn;
-            if (cnt >= QBATCH || (last && cnt > 0)) {
-                // digest the batch: (64-quartet block, component chunk, bra slice, ket slice)
+            for (int b0 = 0; b0 < cnt0 && (cnt0 - b0 >= QB || last); b0 += QB) {
+                // digest the batch qlist[b0, b0+cnt): (64-quartet block, chunk, bra slice, ket slice)
+                const int cnt = min(QB, cnt0 - b0);
                 if (R.stat_nquart && threadIdx.x == 0) atomicAdd(R.stat_nquart, (unsigned long long)cnt);
                 const int nks = (misc[2] + PSLICE - 1) / PSLICE;
                 const int total = ((cnt + 63) / 64) * S::NCHUNK * nbs * nks;
@@ -447,7 +481,7 @@ __global__ void __launch_bounds__(256) eri_fock_class_kernel(ClassArgs A, RowArg
                     const int ks = r % nks;
                     const int idx = (r / nks) * 64 + lane;
                     if (idx >= cnt) continue;
-                    const int Q = qlist[idx];
+                    const int Q = qlist[b0 + idx];
                     const int2 qd = make_int2(R.boff + P, R.koff + Q);
                     const ShellPair ket = A.pairs[qd.y];
                     const int jq0 = ks * PSLICE;
@@ -458,19 +492,58 @@ __global__ void __launch_bounds__(256) eri_fock_class_kernel(ClassArgs A, RowArg
                         atomicAdd(R.stat_nprim, (unsigned long long)(ip1 - ip0) * (unsigned long long)(jq1 - jq0));
                     static_switch<0, S::NCHUNK>(chunk, [&](auto ic) {
                         constexpr int C = decltype(ic)::value;
-                        quartet_item<LA, LB, LC, LD, C * S::CH, (C + 1) * S::CH>(
-                            A, lds_boys, bp, qd, bra, ket, ip0, ip1, jq0, jq1, rows,
-                            rows + (bra.same ? 0 : S::NA * N));
+                        constexpr int A0 = C * S::CH, A1 = (C + 1) * S::CH;
+                        T acc[(A1 - A0) * S::NCD];
+                        quartet_compute<LA, LB, LC, LD, A0, A1, T>(A, lds_boys, bp, ket, ip0, ip1, jq0, jq1, acc);
+                        if constexpr (S::IBUF) {
+                            double* dst = ibuf + (size_t)idx * S::NABCD + A0 * S::NCD;
+                            sfor<(A1 - A0) * S::NCD>(
+                                [&](auto k) { atomicAdd(&dst[decltype(k)::value], (double)acc[k]); });
+                        } else {
+                            double accd[(A1 - A0) * S::NCD];
+                            sfor<(A1 - A0) * S::NCD>([&](auto k) { accd[k] = (double)acc[k]; });
+                            quartet_digest<LA, LB, LC, LD, A0, A1>(A, qd, bra, ket, accd, rowA, rowB);
+                        }
                     });
                 }
-                __syncthreads();
-                if (threadIdx.x == 0) {
-                    misc[1] = 0;
-                    misc[2] = 0;
+                if constexpr (S::IBUF) {
+                    // every quartet of the batch once: its summed integrals into G
+                    __syncthreads();
+                    const int total2 = ((cnt + 63) / 64) * S::NCHUNK;
+                    for (int w = wv; w < total2; w += nwv) {
+                        const int chunk = w % S::NCHUNK;
+                        const int idx = (w / S::NCHUNK) * 64 + lane;
+                        if (idx >= cnt) continue;
+                        const int Q = qlist[b0 + idx];
+                        const int2 qd = make_int2(R.boff + P, R.koff + Q);
+                        const ShellPair ket = A.pairs[qd.y];
+                        static_switch<0, S::NCHUNK>(chunk, [&](auto ic) {
+                            constexpr int C = decltype(ic)::value;
+                            constexpr int A0 = C * S::CH, A1 = (C + 1) * S::CH;
+                            double acc[(A1 - A0) * S::NCD];
+                            double* src = ibuf + (size_t)idx * S::NABCD + A0 * S::NCD;
+                            sfor<(A1 - A0) * S::NCD>([&](auto k) {
+                                acc[k] = src[decltype(k)::value];
+                                src[decltype(k)::value] = 0.0;
+                            });
+                            quartet_digest<LA, LB, LC, LD, A0, A1>(A, qd, bra, ket, acc, rowA, rowB);
+                        });
+                    }
                 }
                 __syncthreads();
             }
             if (last) break;
+            // keep the survivors that did not fill a batch at the front of the list
+            const int done = (cnt0 / QB) * QB;
+            const int rest = cnt0 - done;
+            if (done > 0) {
+                int v = 0;
+                if ((int)threadIdx.x < rest) v = qlist[done + threadIdx.x];
+                __syncthreads();
+                if ((int)threadIdx.x < rest) qlist[threadIdx.x] = v;
+                if (threadIdx.x == 0) misc[1] = rest;
+            }
+            __syncthreads();
         }
         __syncthreads();
         for (int i = threadIdx.x; i < nrows * N; i += blockDim.x) {
@@ -482,12 +555,13 @@ __global__ void __launch_bounds__(256) eri_fock_class_kernel(ClassArgs A, RowArg
     }
 }
 
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, typename T>
 inline size_t fock_class_lds_bytes(int N, int maxpp)
 {
     using S = ClassShape<LA, LB, LC, LD>;
-    return sizeof(double) * (BOYS_TAYLOR * BOYS_NT + (size_t)(S::NA + S::NB) * N + (size_t)maxpp * S::BPS)
-         + sizeof(int) * (QLIST + 4);
+    return sizeof(double) * ((size_t)(S::NA + S::NB) * N + S::IBUF_DOUBLES)
+         + sizeof(T) * (BOYS_TAYLOR * BOYS_NT + (size_t)maxpp * S::BPS)
+         + sizeof(int) * (ClassList<S::QB>::QLIST + 4);
 }
 
 }  // namespace mmd

@@ -22,13 +22,13 @@ inline __host__ __device__ void qclass_decode(int qc, int* la, int* lb, int* lc,
     *ld = pck - l3 * (l3 + 1) / 2;
 }
 
-hipError_t launch_fock_class(int qc, const ClassArgs& A, const RowArgs& R, int grid, hipStream_t st);
-size_t fock_class_lds(int qc, int N, int maxpp);
+// fp32 = 1 selects the FP32-tier instantiation of the class kernel.
+hipError_t launch_fock_class(int qc, int fp32, const ClassArgs& A, const RowArgs& R, int grid, hipStream_t st);
+size_t fock_class_lds(int qc, int fp32, int N, int maxpp);
 hipError_t launch_fock_wave_class(int la, int lb, int lc, int ld, const ClassArgs& A, int64_t max_quartets,
                                   hipStream_t st);
 // Classes served by the wavefront-per-quartet kernel instead of a unrolled class kernel.
 inline bool fock_class_is_wave(int la, int lb, int lc, int ld) { return la == 2 && lb == 2 && lc == 2 && ld == 2; }
-int fock_class_nchunk(int qc);
 
 // A single Cartesian basis function for the generic (per basis-function
 // quartet) kernel: centre, exponents (l,m,n), primitive range; the

@@ -35,6 +35,7 @@ struct ClassArgs {
     const ShellPair* pairs;     // all shell pairs
     const PrimPair* pp;         // primitive pairs
     const double* boys;         // Boys table [BOYS_ROWS][BOYS_NT]
+    const float* boysf;         // the same table in float (FP32-tier kernels)
     const double* gcomp;        // per-basis-function component scale
     const double* D;            // density (difference) N x N
     const double* absD;         // |dP| N x N (screening)
@@ -53,6 +54,7 @@ struct RowArgs {
     int maxpp;                  // largest primitive-pair count of the class's bra pairs
     int nshell;
     const double* Dsh;          // shell-block maxima of |dP| (nshell x nshell)
+    double tier_lo, tier_hi;    // this launch takes quartets with tier_lo <= bound < tier_hi
     int* row_counter;           // next work unit (zeroed before the launch)
     unsigned long long* stat_nquart;   // optional statistics
     unsigned long long* stat_nprim;

@@ -70,12 +70,34 @@ lib.mmd_fock_stats.restype = _c_int
 lib.mmd_fock_stats.argtypes = [_ctx_p, _vp, _vp, _vp, _vp]
 lib.mmd_last_fock_ms.restype = _c_int
 lib.mmd_last_fock_ms.argtypes = [_ctx_p, ctypes.POINTER(_c_double)]
+lib.mmd_set_precision.restype = _c_int
+lib.mmd_set_precision.argtypes = [_ctx_p, _c_double]
+lib.mmd_fock_tiers.restype = _c_int
+lib.mmd_fock_tiers.argtypes = [_ctx_p, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64)]
+lib.mmd_mlp_create.restype = _c_int
+lib.mmd_mlp_create.argtypes = [ctypes.POINTER(_vp), _c_int, _vp, _c_int]
+lib.mmd_mlp_destroy.restype = None
+lib.mmd_mlp_destroy.argtypes = [_vp]
+lib.mmd_mlp_nparam.restype = _c_i64
+lib.mmd_mlp_nparam.argtypes = [_vp]
+lib.mmd_mlp_set_params.restype = _c_int
+lib.mmd_mlp_set_params.argtypes = [_vp, _vp]
+lib.mmd_mlp_get_params.restype = _c_int
+lib.mmd_mlp_get_params.argtypes = [_vp, _vp]
+lib.mmd_mlp_forward.restype = _c_int
+lib.mmd_mlp_forward.argtypes = [_vp, _c_int, _vp, _vp]
+lib.mmd_mlp_loss_grad.restype = _c_int
+lib.mmd_mlp_loss_grad.argtypes = [_vp, _c_int, _vp, _vp, _vp, _vp]
+lib.mmd_mlp_train.restype = _c_int
+lib.mmd_mlp_train.argtypes = [_vp, _c_int, _vp, _vp, _c_int, _c_int, ctypes.c_float, ctypes.c_uint64, _vp]
 
 EXPORTED = [
     "mmd_last_error", "mmd_version", "mmd_device_count", "mmd_ctx_create", "mmd_ctx_destroy",
     "mmd_set_basis", "mmd_set_atoms", "mmd_one_electron", "mmd_schwarz", "mmd_set_schwarz",
     "mmd_eri_tensor", "mmd_eri_list", "mmd_fock_delta", "mmd_jk_tensor", "mmd_boys",
-    "mmd_last_fock_ms", "mmd_set_profiling", "mmd_fock_stats",
+    "mmd_last_fock_ms", "mmd_set_profiling", "mmd_fock_stats", "mmd_set_precision", "mmd_fock_tiers",
+    "mmd_mlp_create", "mmd_mlp_destroy", "mmd_mlp_nparam", "mmd_mlp_set_params", "mmd_mlp_get_params",
+    "mmd_mlp_forward", "mmd_mlp_loss_grad", "mmd_mlp_train",
 ]
 
 
@@ -114,6 +136,7 @@ class Context:
         h = _ctx_p()
         check(lib.mmd_ctx_create(ctypes.byref(h), int(device)))
         self._h = h
+        self.device = int(device)
         self.nbf = 0
 
     def close(self):
@@ -249,6 +272,87 @@ class Context:
         v = _c_double(0.0)
         check(lib.mmd_last_fock_ms(self._h, ctypes.byref(v)))
         return v.value
+
+    def set_precision(self, fp32_below=0.0):
+        """FP32 tier: shell quartets with screening bound below fp32_below
+        get FP32 integrals (FP64 digestion); 0 keeps everything FP64."""
+        check(lib.mmd_set_precision(self._h, float(fp32_below)))
+
+    def fock_tiers(self):
+        """(surviving shell quartets, of which FP32) of the last symmetrised build."""
+        a, b = _c_i64(0), _c_i64(0)
+        check(lib.mmd_fock_tiers(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+
+class MLP:
+    """The learned guess's perceptron on the GPU (mmd_mlp_* of include/mmd.h):
+    tanh hidden layers, linear output, FP32."""
+
+    def __init__(self, dims, device=0):
+        self.dims = [int(d) for d in dims]
+        h = _vp()
+        d = np.ascontiguousarray(self.dims, dtype=np.int32)
+        check(lib.mmd_mlp_create(ctypes.byref(h), len(self.dims) - 1, _ptr(d), device))
+        self._h = h
+        self.nparam = int(lib.mmd_mlp_nparam(h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.mmd_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, flat):
+        flat = np.ascontiguousarray(flat, dtype=np.float32)
+        if flat.size != self.nparam:
+            raise ValueError("expected %d parameters, got %d" % (self.nparam, flat.size))
+        check(lib.mmd_mlp_set_params(self._h, _ptr(flat)))
+
+    def get_params(self):
+        out = np.empty(self.nparam, dtype=np.float32)
+        check(lib.mmd_mlp_get_params(self._h, _ptr(out)))
+        return out
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        if X.ndim != 2 or X.shape[1] != self.dims[0]:
+            raise ValueError("input must be (n, %d)" % self.dims[0])
+        Y = np.empty((X.shape[0], self.dims[-1]), dtype=np.float32)
+        check(lib.mmd_mlp_forward(self._h, X.shape[0], _ptr(X), _ptr(Y)))
+        return Y
+
+    def loss_grad(self, X, Yt):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Yt = np.ascontiguousarray(Yt, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        g = np.empty(self.nparam, dtype=np.float32)
+        check(lib.mmd_mlp_loss_grad(self._h, X.shape[0], _ptr(X), _ptr(Yt), _ptr(loss), _ptr(g)))
+        return float(loss[0]), g
+
+    def train(self, X, Yt, epochs, batch=1024, lr=1e-3, seed=0):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Yt = np.ascontiguousarray(Yt, dtype=np.float32)
+        losses = np.zeros(max(1, epochs), dtype=np.float32)
+        check(lib.mmd_mlp_train(self._h, X.shape[0], _ptr(X), _ptr(Yt), int(epochs), int(batch), float(lr),
+                                int(seed), _ptr(losses)))
+        return losses[:epochs]
+
+
+def mlp_forward(ctx, model, X):
+    """Forward pass of a mmd.learned.Model on the GPU of ``ctx``."""
+    dev = getattr(ctx, "device", 0)
+    net = getattr(model, "_gpu", None)
+    if net is None or net.dims != model.dims:
+        net = MLP(model.dims, dev)
+        net.set_params(model.flat())
+        model._gpu = net
+    return net.forward(X)
 
 
 _default = {}

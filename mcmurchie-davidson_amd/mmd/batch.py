@@ -1,0 +1,224 @@
+"""Many independent RHF calculations on one GPU.
+
+``run_batch`` runs ``Molecule(geometry, basis).RHF(...)`` for a list of
+geometries with a pool of worker threads.  Each worker owns one libmmd
+context (its own streams and device buffers) and reuses it for every
+molecule it takes, so that the Fock builds of different molecules overlap
+on the GPU while other workers diagonalise on the host cores (the library
+calls and LAPACK release the GIL).  For small molecules the SCF driver's
+Python code itself is the bottleneck, so ``run_batch_procs`` spreads the
+batch over several worker processes (each with its own threads and GPU
+contexts); ``autotune_workers`` / ``autotune_procs`` pick the pool shape by
+timing a short calibration batch.
+
+The SCF of every molecule is exactly ``SCF.RHF`` (mmd/scf.py:10-80 of the
+reference); batching changes only what runs concurrently.
+"""
+import concurrent.futures as cf
+import json
+import os
+import subprocess
+import sys
+import threading
+import time
+
+import numpy as np
+
+from mmd import _native
+from mmd.molecule import Molecule
+from mmd.scf import _blas_threads
+
+
+def _worker_ctx(local, device):
+    ctx = getattr(local, "ctx", None)
+    if ctx is None:
+        ctx = _native.Context(device)
+        local.ctx = ctx
+    return ctx
+
+
+def _run_one(geom, basis, kwargs, local, device, keep_density):
+    mol = Molecule(geom, basis)
+    mol.attach_context(_worker_ctx(local, device))
+    t0 = time.perf_counter()
+    mol._rhf(False, kwargs.get("DIIS", True), kwargs.get("direct", True), kwargs.get("conver", 1e-8),
+             kwargs.get("acc2e", 1e-12), kwargs.get("guess", "core"))
+    out = {"energy": float(mol.energy.real), "iterations": int(mol.iterations),
+           "converged": bool(mol.is_converged), "nbasis": int(mol.nbasis),
+           "seconds": time.perf_counter() - t0}
+    if keep_density:
+        out["P"] = np.real(mol.P).copy()
+    return out
+
+
+def run_batch(geometries, basis="6-31g", workers=8, device=0, keep_density=False, executor=None, **rhf_kwargs):
+    """RHF energies of all geometries.  rhf_kwargs: DIIS, direct (default
+    True), conver, acc2e, guess.  Returns one dict per geometry, in order:
+    energy, iterations, converged, nbasis, seconds (and the converged density
+    P with keep_density).  ``executor``: a thread pool to reuse (its
+    threads keep their GPU contexts between calls)."""
+    if workers < 1:
+        raise ValueError("workers must be >= 1")
+    with _blas_threads(1):
+        if executor is not None:
+            futs = [executor.submit(_run_one, g, basis, rhf_kwargs, _TLS, device, keep_density) for g in geometries]
+            return [f.result() for f in futs]
+        local = threading.local()
+        with cf.ThreadPoolExecutor(max_workers=workers) as ex:
+            futs = [ex.submit(_run_one, g, basis, rhf_kwargs, local, device, keep_density) for g in geometries]
+            return [f.result() for f in futs]
+
+
+_TLS = threading.local()   # per-thread contexts of a reused executor
+
+
+class BatchPool:
+    """Persistent worker processes (``threads`` threads each, with their GPU
+    contexts kept between calls) that solve batches of geometries.  Start it
+    from a process that has not used the GPU yet."""
+
+    def __init__(self, procs=4, threads=2, device=0):
+        if procs < 1 or threads < 1:
+            raise ValueError("procs and threads must be >= 1")
+        pkg = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+        env = dict(os.environ)
+        env["PYTHONPATH"] = pkg + (os.pathsep + env["PYTHONPATH"] if env.get("PYTHONPATH") else "")
+        self.procs, self.threads = procs, threads
+        self._ps = [subprocess.Popen([sys.executable, "-m", "mmd.batch", "--serve", str(threads), str(device)],
+                                     stdin=subprocess.PIPE, stdout=subprocess.PIPE, env=env) for _ in range(procs)]
+
+    def run(self, geometries, basis="6-31g", **rhf_kwargs):
+        chunks = [list(range(i, len(geometries), self.procs)) for i in range(self.procs)]
+        for c, p in zip(chunks, self._ps):
+            p.stdin.write((json.dumps({"geoms": [geometries[j] for j in c], "basis": basis,
+                                       "kwargs": rhf_kwargs}) + "\n").encode())
+            p.stdin.flush()
+        out = [None] * len(geometries)
+        for c, p in zip(chunks, self._ps):
+            line = p.stdout.readline()
+            if not line:
+                raise RuntimeError("batch worker exited (status %s)" % p.poll())
+            for j, r in zip(c, json.loads(line.decode())):
+                out[j] = r
+        return out
+
+    def close(self):
+        for p in self._ps:
+            try:
+                p.stdin.close()
+            except Exception:
+                pass
+        for p in self._ps:
+            p.wait()
+        self._ps = []
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def _serve_main(threads, device):
+    ex = cf.ThreadPoolExecutor(max_workers=threads)
+    for line in sys.stdin:
+        job = json.loads(line)
+        res = run_batch(job["geoms"], job["basis"], workers=threads, device=device, executor=ex, **job["kwargs"])
+        sys.stdout.write(json.dumps(res) + "\n")
+        sys.stdout.flush()
+    ex.shutdown()
+
+
+def autotune_workers(geometries, basis="6-31g", candidates=(4, 8, 16), device=0, **rhf_kwargs):
+    """Pool size with the highest SCF/s on the calibration geometries, and
+    the measured rates {workers: SCF/s}."""
+    rates = {}
+    for w in candidates:
+        t0 = time.perf_counter()
+        run_batch(geometries, basis, workers=w, device=device, **rhf_kwargs)
+        rates[w] = len(geometries) / (time.perf_counter() - t0)
+    best = max(rates, key=rates.get)
+    return best, rates
+
+
+def _spawn(chunk, basis, threads, device, rhf_kwargs, pairs_path):
+    pkg = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ)
+    env["PYTHONPATH"] = pkg + (os.pathsep + env["PYTHONPATH"] if env.get("PYTHONPATH") else "")
+    p = subprocess.Popen([sys.executable, "-m", "mmd.batch", "--worker"], stdin=subprocess.PIPE,
+                         stdout=subprocess.PIPE, env=env)
+    p.stdin.write(json.dumps({"geoms": chunk, "basis": basis, "threads": threads, "device": device,
+                              "kwargs": rhf_kwargs, "pairs": pairs_path}).encode())
+    p.stdin.close()
+    return p
+
+
+def run_batch_procs(geometries, basis="6-31g", procs=8, threads=2, device=0, pairs_dir=None, **rhf_kwargs):
+    """``run_batch`` spread over ``procs`` worker processes of ``threads``
+    threads each (molecules dealt round-robin).  Call it from a process that
+    has not used the GPU yet: the workers are started as new programs.
+    With pairs_dir, every worker also writes the learned guess's training
+    pairs of its converged molecules to pairs_dir/pairs_<k>.npz (arrays X, Y
+    and mol, the molecule index of each row)."""
+    if procs < 1 or threads < 1:
+        raise ValueError("procs and threads must be >= 1")
+    procs = min(procs, len(geometries)) or 1
+    chunks = [list(range(i, len(geometries), procs)) for i in range(procs)]
+    ps = [_spawn([geometries[j] for j in c], basis, threads, device, rhf_kwargs,
+                 os.path.join(pairs_dir, "pairs_%d.npz" % k) if pairs_dir else None) for k, c in enumerate(chunks)]
+    out = [None] * len(geometries)
+    err = None
+    for c, p in zip(chunks, ps):
+        data = p.stdout.read()
+        rc = p.wait()
+        if rc != 0:
+            err = err or RuntimeError("batch worker exited with status %d" % rc)
+            continue
+        for j, r in zip(c, json.loads(data.decode())):
+            out[j] = r
+    if err:
+        raise err
+    return out
+
+
+def autotune_procs(geometries, basis="6-31g", candidates=((4, 2), (8, 2), (12, 1)), device=0, **rhf_kwargs):
+    """(procs, threads) with the highest SCF/s on the calibration geometries,
+    and the measured rates."""
+    rates = {}
+    for pt in candidates:
+        t0 = time.perf_counter()
+        run_batch_procs(geometries, basis, procs=pt[0], threads=pt[1], device=device, **rhf_kwargs)
+        rates[pt] = len(geometries) / (time.perf_counter() - t0)
+    best = max(rates, key=rates.get)
+    return best, rates
+
+
+def _worker_main():
+    job = json.loads(sys.stdin.read())
+    pairs = job.get("pairs")
+    res = run_batch(job["geoms"], job["basis"], workers=job["threads"], device=job["device"],
+                    keep_density=bool(pairs), **job["kwargs"])
+    if pairs:
+        from mmd.learned import training_pair
+        X, Y, M = [], [], []
+        for i, (g, r) in enumerate(zip(job["geoms"], res)):
+            P = r.pop("P")
+            if not r["converged"]:
+                continue
+            f, t = training_pair(Molecule(g, job["basis"]), P)
+            X.append(f)
+            Y.append(t)
+            M.append(np.full(len(f), i))
+        np.savez(pairs, X=np.concatenate(X), Y=np.concatenate(Y), mol=np.concatenate(M))
+    sys.stdout.write(json.dumps(res))
+    sys.stdout.flush()
+
+
+def batch_energies(results):
+    return np.array([r["energy"] for r in results])
+
+
+if __name__ == "__main__" and "--worker" in sys.argv:
+    _worker_main()
+elif __name__ == "__main__" and "--serve" in sys.argv:
+    _serve_main(int(sys.argv[sys.argv.index("--serve") + 1]), int(sys.argv[sys.argv.index("--serve") + 2]))

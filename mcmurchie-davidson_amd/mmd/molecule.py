@@ -73,7 +73,8 @@ class Molecule(SCF):
         self.is_built = False
         self.geometry_input = geometry
         self._ctx = None
-        basis_file = os.path.join(os.path.dirname(__file__), 'basis', str(basis).lower() + '.gbs')
+        self.basis_name = str(basis).lower()
+        basis_file = os.path.join(os.path.dirname(__file__), 'basis', self.basis_name + '.gbs')
         self.basis_data = self.getBasis(basis_file)
         self.formBasis()
 
@@ -88,6 +89,16 @@ class Molecule(SCF):
             _native.register_bfs_context(self.bfs, ctx)
             self._ctx = ctx
         return self._ctx
+
+    def attach_context(self, ctx):
+        """Use an existing libmmd context (e.g. one reused across a batch of
+        molecules): this molecule's basis and nuclei replace its contents."""
+        ctx.set_basis_from_bfs(self.bfs)
+        ctx.set_atoms([a.charge for a in self.atoms], np.array([a.origin for a in self.atoms]))
+        ctx._screen_key = None
+        _native.register_bfs_context(self.bfs, ctx)
+        self._ctx = ctx
+        return ctx
 
     @property
     def _forces(self):

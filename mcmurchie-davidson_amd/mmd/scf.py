@@ -38,13 +38,21 @@ def _re(A):
 class SCF(object):
     """SCF methods and routines for a Molecule."""
 
-    def RHF(self, doPrint=True, DIIS=True, direct=False, conver=1e-8, acc2e=1e-12):
+    def RHF(self, doPrint=True, DIIS=True, direct=False, conver=1e-8, acc2e=1e-12, guess='core'):
         """Closed-shell SCF (mmd/scf.py:10-80).  Converged when
-        ||P - P_old||_F < conver; at most 100 iterations."""
-        with _blas_threads(1 if self.nbasis < 600 else 4):
-            return self._rhf(doPrint, DIIS, direct, conver, acc2e)
+        ||P - P_old||_F < conver; at most 100 iterations.
 
-    def _rhf(self, doPrint, DIIS, direct, conver, acc2e):
+        guess='core' starts from the core Hamiltonian as the reference does;
+        guess='sad' starts from the superposition of atomic densities
+        (mmd.guess.sad_density): F = H + G(P_sad), and the first incremental
+        build is G(P_1 - P_sad); guess='learned' starts from SAD corrected
+        per water monomer by the trained perceptron (mmd.learned)."""
+        if guess not in ('core', 'sad', 'learned'):
+            raise ValueError("guess must be 'core', 'sad' or 'learned'")
+        with _blas_threads(1 if self.nbasis < 600 else 4):
+            return self._rhf(doPrint, DIIS, direct, conver, acc2e, guess)
+
+    def _rhf(self, doPrint, DIIS, direct, conver, acc2e, guess='core'):
         self.is_converged = False
         self.delta_energy = 1e20
         self.P_RMS = 1e20
@@ -59,6 +67,16 @@ class SCF(object):
 
         self.P = self.P_old
         self.F = self.Core.astype('complex')
+        if guess in ('sad', 'learned'):
+            if guess == 'sad':
+                from mmd.guess import sad_density
+                self.P = sad_density(self).astype('complex')
+            else:
+                from mmd.learned import learned_density
+                self.P = learned_density(self, getattr(self, 'guess_model', None)).astype('complex')
+            self.F_old = self.F
+            self.buildFock()          # F = H + G(P_sad)
+            self.P_old = self.P
 
         if DIIS:
             self.fockSet = []

@@ -281,3 +281,143 @@ def test_rhf_water_ccpvdz_direct_vs_conventional_oracle(native, oracle):
     mol.RHF(doPrint=False, direct=True, conver=1e-10)
     ref = oracle.rhf(oracle.OracleMolecule(WATER, _gbs("cc-pvdz")), direct=False, conver=1e-10)
     assert abs(mol.energy.real - ref["energy"].real) < 1e-8
+
+
+# --------------------------------------------------------------------------
+# SAD initial guess (no reference counterpart): same converged energy as the
+# reference's core guess, within 1e-8 Eh, in fewer iterations.
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("geom,basis", [(WATER, "cc-pvdz"), (water_cluster(3, 6.0, 4), "6-31g")])
+def test_sad_guess_matches_core_guess(native, geom, basis):
+    from mmd.molecule import Molecule
+    core = Molecule(geom, basis)
+    core.RHF(doPrint=False, direct=True, conver=1e-10)
+    sad = Molecule(geom, basis)
+    sad.RHF(doPrint=False, direct=True, conver=1e-10, guess="sad")
+    print("core %d iterations, SAD %d" % (core.iterations, sad.iterations))
+    assert core.is_converged and sad.is_converged
+    assert abs(sad.energy.real - core.energy.real) < 1e-8
+    assert sad.iterations < core.iterations
+
+
+# --------------------------------------------------------------------------
+# Batched SCF (no reference counterpart): every molecule's energy equals its
+# own single-molecule run (1e-10 Eh) and, for a (H2O)2 member, the oracle's
+# conventional RHF (1e-8 Eh).
+# --------------------------------------------------------------------------
+def test_batch_matches_single_runs_and_oracle(native, oracle):
+    from mmd.batch import run_batch
+    from mmd.molecule import Molecule
+    geoms = [water_cluster(n, 6.0, 100 + i, perturb=True) for i, n in enumerate([2, 3, 2, 4, 3, 2])]
+    res = run_batch(geoms, "6-31g", workers=3, conver=1e-10)
+    for g, r in zip(geoms, res):
+        mol = Molecule(g, "6-31g")
+        mol.RHF(doPrint=False, direct=True, conver=1e-10)
+        assert r["converged"] and abs(r["energy"] - mol.energy.real) < 1e-10
+        assert r["iterations"] == mol.iterations
+    ref = oracle.rhf(oracle.OracleMolecule(geoms[0], _gbs("6-31g")), direct=False, conver=1e-10)
+    assert abs(res[0]["energy"] - ref["energy"].real) < 1e-8
+
+
+# --------------------------------------------------------------------------
+# FP32 tier (no reference counterpart): quartets with screening bound below
+# 1e-4 take FP32 integrals.  The Fock matrix stays within 1e-7 of FP64 and
+# the converged energy within 1 uEh; fp32_below = 0 is bit-for-bit FP64.
+# --------------------------------------------------------------------------
+def test_fp32_tier_fock_close_to_fp64(native):
+    from mmd.molecule import Molecule
+    mol = Molecule(water_cluster(4, 6.0, 7), "cc-pvdz")
+    mol.build(direct=True)
+    ctx = mol.native
+    rng = np.random.default_rng(3)
+    dP = _sym_random(mol.nbasis, rng, 0.05)
+    G64, _, n64 = ctx.fock_delta(dP, None, tol=1e-12)
+    ctx.set_precision(1e-4)
+    G32, _, n32 = ctx.fock_delta(dP, None, tol=1e-12)
+    ntot, nfp32 = ctx.fock_tiers()
+    ctx.set_precision(0.0)
+    G64b, _, _ = ctx.fock_delta(dP, None, tol=1e-12)
+    print("FP32 tier: %d of %d shell quartets, max |dG| %.2e" % (nfp32, ntot, np.max(np.abs(G32 - G64))))
+    assert n32 == n64 == ntot and 0 < nfp32 < ntot
+    assert np.max(np.abs(G32 - G64)) < 1e-7
+    assert np.max(np.abs(G64b - G64)) < 1e-12
+    with pytest.raises(ValueError):
+        ctx.set_precision(-1.0)
+
+
+def test_fp32_tier_energy_within_microhartree(native):
+    from mmd.molecule import Molecule
+    geom = water_cluster(4, 6.0, 11)
+    ref = Molecule(geom, "cc-pvdz")
+    ref.RHF(doPrint=False, direct=True, conver=1e-8)
+    mix = Molecule(geom, "cc-pvdz")
+    mix.native.set_precision(1e-4)
+    mix.RHF(doPrint=False, direct=True, conver=1e-8)
+    print("FP64 %.10f  tier %.10f  (%d / %d iterations)" % (ref.energy.real, mix.energy.real, ref.iterations,
+                                                            mix.iterations))
+    assert mix.is_converged
+    assert abs(mix.energy.real - ref.energy.real) < 1e-6
+
+
+# --------------------------------------------------------------------------
+# Perceptron of the learned guess (no reference counterpart): HIP forward
+# and backward against the numpy restatement (FP32: 1e-4 relative), and
+# training by the HIP Adam loop.
+# --------------------------------------------------------------------------
+def test_mlp_forward_and_gradient_vs_numpy(native):
+    import mlp_ref
+    dims = [15, 32, 24, 91]
+    net = native.MLP(dims)
+    rng = np.random.default_rng(8)
+    flat = (rng.standard_normal(net.nparam) * 0.3).astype(np.float32)
+    net.set_params(flat)
+    assert np.array_equal(net.get_params(), flat)
+    X = rng.standard_normal((300, 15)).astype(np.float32)
+    Y = rng.standard_normal((300, 91)).astype(np.float32)
+    ref = mlp_ref.forward(flat.astype(np.float64), dims, X.astype(np.float64))[-1]
+    assert np.max(np.abs(net.forward(X) - ref)) < 1e-4 * np.max(np.abs(ref))
+    loss, g = net.loss_grad(X, Y)
+    loss_ref, g_ref = mlp_ref.loss_grad(flat.astype(np.float64), dims, X.astype(np.float64), Y.astype(np.float64))
+    assert abs(loss - loss_ref) < 1e-5 * loss_ref
+    assert np.max(np.abs(g - g_ref)) < 1e-4 * np.max(np.abs(g_ref))
+    with pytest.raises(ValueError):
+        net.forward(np.zeros((3, 14), dtype=np.float32))
+
+
+def test_mlp_training_fits_smooth_map(native):
+    import mlp_ref
+    rng = np.random.default_rng(9)
+    dims = [4, 32, 32, 3]
+    X = rng.uniform(-1, 1, (4096, 4)).astype(np.float32)
+    Y = np.stack([np.sin(X[:, 0] + X[:, 1]), X[:, 2] * X[:, 3], np.cos(2 * X[:, 0]) * X[:, 3]], axis=1)
+    net = native.MLP(dims)
+    nparam = net.nparam
+    net.set_params((rng.standard_normal(nparam) * 0.3).astype(np.float32))
+    losses = net.train(X, Y.astype(np.float32), epochs=60, batch=256, lr=3e-3, seed=1)
+    assert losses.shape == (60,) and losses[-1] < 0.1 * losses[0]
+    final = mlp_ref.loss_grad(net.get_params().astype(np.float64), dims, X.astype(np.float64), Y)[0]
+    assert final < 2 * losses[-1] + 1e-4
+
+
+# --------------------------------------------------------------------------
+# Learned guess (no reference counterpart): on held-out water clusters
+# (generator seed 2, as the training script draws them) the learned start
+# converges to the core-guess energy within 1e-8 Eh in fewer iterations
+# than SAD in total.
+# --------------------------------------------------------------------------
+def test_learned_guess_gain_on_heldout(native):
+    from mmd.molecule import Molecule
+    rng = np.random.default_rng(2)
+    ns = rng.integers(2, 7, size=6)
+    geoms = [water_cluster(int(k), 2.0 + 1.4 * int(k), seed=2 * 1000003 + i, perturb=True) for i, k in enumerate(ns)]
+    it = {"sad": 0, "learned": 0}
+    for g in geoms:
+        core = Molecule(g, "6-31g")
+        core.RHF(doPrint=False, direct=True, conver=1e-10)
+        for guess in ("sad", "learned"):
+            m = Molecule(g, "6-31g")
+            m.RHF(doPrint=False, direct=True, conver=1e-10, guess=guess)
+            assert m.is_converged and abs(m.energy.real - core.energy.real) < 1e-8
+            it[guess] += m.iterations
+    print("iterations over %d held-out clusters: %s" % (len(geoms), it))
+    assert it["learned"] < it["sad"]

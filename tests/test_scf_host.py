@@ -111,3 +111,62 @@ def test_diis_singular_subspace_raises(cpu_molecule):
     mol.updateDIIS(F, P)
     with pytest.raises((np.linalg.LinAlgError, AssertionError)):
         mol.updateDIIS(F, P)
+
+
+# -- SAD initial guess ----------------------------------------------------------
+
+@pytest.fixture
+def oracle_atoms(monkeypatch, oracle):
+    """Atomic problems of the SAD guess on the oracle-backed stand-in."""
+    import mmd.guess as guess
+    from mmd.molecule import Molecule
+
+    def make(symbol, basis):
+        geom = "0 1\n%s 0.0 0.0 0.0\n" % symbol
+        a = Molecule(geom, basis)
+        a._ctx = OracleBackedContext(oracle.OracleMolecule(geom, os.path.join(BASIS, basis + ".gbs")))
+        a.one_electron_integrals()
+        a.two_electron_integrals()
+        return a
+    monkeypatch.setattr(guess, "_default_atom_factory", make)
+    monkeypatch.setattr(guess, "_ATOM_CACHE", {})
+    return guess
+
+
+def test_occupations_spherical_average(oracle_atoms):
+    """Aufbau with a partly filled degenerate level shared equally (O: 1s2 2s2 2p4)."""
+    occ = oracle_atoms._occupations(np.array([-20.0, -1.2, -0.6, -0.6, -0.6, 0.4]), 8)
+    assert np.allclose(occ, [2, 2, 4 / 3, 4 / 3, 4 / 3, 0])
+    assert np.allclose(oracle_atoms._occupations(np.array([-0.5, 0.2]), 1), [1, 0])
+
+
+def test_sad_density_trace_and_symmetry(cpu_molecule, oracle_atoms):
+    """tr(P_sad S) = nelec/2 (each atomic block holds Z/2 in the reference's
+    half-density convention), P symmetric, equal occupation of O's p components."""
+    mol, _ = cpu_molecule(WATER, "3-21g")
+    mol.build()
+    P = oracle_atoms.sad_density(mol)
+    assert np.allclose(P, P.T)
+    assert abs(np.trace(P @ mol.S) - mol.nelec / 2) < 1e-8
+    pdiag = [P[i, i] for i, b in enumerate(mol.bfs[:9]) if sum(b.shell) == 1]
+    assert len(pdiag) == 6 and np.allclose(pdiag[0:3], pdiag[0]) and np.allclose(pdiag[3:6], pdiag[3])
+
+
+@pytest.mark.parametrize("geom,basis,direct", [(WATER, "3-21g", True), (WATER, "3-21g", False),
+                                               (METHANE, "sto-3g", True)])
+def test_sad_guess_same_energy_fewer_iterations(cpu_molecule, oracle_atoms, geom, basis, direct):
+    """The SAD start converges to the core-guess energy (1e-8) in no more iterations."""
+    mol, _ = cpu_molecule(geom, basis)
+    mol.RHF(doPrint=False, direct=direct, conver=1e-10)
+    e_core, it_core = mol.energy.real, mol.iterations
+    mol2, _ = cpu_molecule(geom, basis)
+    mol2.RHF(doPrint=False, direct=direct, conver=1e-10, guess="sad")
+    assert mol2.is_converged
+    assert abs(mol2.energy.real - e_core) < 1e-8
+    assert mol2.iterations <= it_core
+
+
+def test_rhf_rejects_unknown_guess(cpu_molecule):
+    mol, _ = cpu_molecule(WATER, "sto-3g")
+    with pytest.raises(ValueError):
+        mol.RHF(doPrint=False, guess="huckel")

@@ -27,6 +27,17 @@ for _p in (ROOT, PKG):
 
 METRIC = "converged SCF/sec at |ΔE|≤1 µEh vs plain FP64 GPU; Fock-build ms; quartets/s"
 FP64_PEAK_TFLOPS = 78.6
+# HBM bytes per launch of the class kernels on this workload, from rocprofv3
+# --pmc FETCH_SIZE and WRITE_SIZE runs of this benchmark, scaled by the same
+# counters read for a copy of known size (recorded with the command lines in
+# profiles/); a class absent here reports null.
+TRAFFIC = {
+    (0, 0, 0, 0): 2.228e+07, (1, 0, 0, 0): 4.036e+07, (1, 0, 1, 0): 4.495e+07, (1, 1, 0, 0): 2.102e+07,
+    (1, 1, 1, 0): 5.383e+07, (1, 1, 1, 1): 5.818e+07, (2, 0, 0, 0): 1.944e+07, (2, 0, 1, 0): 3.816e+07,
+    (2, 0, 1, 1): 3.418e+07, (2, 0, 2, 0): 1.049e+07, (2, 1, 0, 0): 1.522e+07, (2, 1, 1, 0): 4.247e+07,
+    (2, 1, 1, 1): 4.255e+07, (2, 1, 2, 0): 2.125e+07, (2, 1, 2, 1): 2.862e+07, (2, 2, 0, 0): 5.785e+06,
+    (2, 2, 1, 0): 1.233e+07, (2, 2, 1, 1): 1.428e+07, (2, 2, 2, 0): 7.647e+06, (2, 2, 2, 1): 1.919e+07,
+}
 
 
 def ncart(l):
@@ -43,11 +54,11 @@ def class_flops(la, lb, lc, ld):
     return 2 * H(Lab) * H(Lcd) * Ccd + 2 * Cab * H(Lab) * Ccd + 3 * NR(Lab + Lcd) + 40
 
 
-def run_scf(geom, basis, profile=False):
+def run_scf(geom, basis, profile=False, serial=False):
     from mmd.molecule import Molecule
     mol = Molecule(geom, basis)
     if profile:
-        mol.native.set_profiling(True)
+        mol.native.set_profiling(True, serial=serial)
         mol._profile_stats = []
         orig = mol.native.fock_delta
 
@@ -175,7 +186,7 @@ def bench_batch(args):
     # autotune the pool shape on a warm pool and a 64-molecule calibration
     # batch; worker processes start before this process uses the GPU
     rates = {}
-    for shape in ((4, 2), (4, 3), (4, 4), (5, 2)):
+    for shape in ((4, 2), (6, 2), (8, 2), (8, 1), (12, 1)):
         with BatchPool(*shape) as pool:
             pool.run(geoms[:2 * shape[0] * shape[1]], "6-31g")        # warm-up: contexts, caches
             t0 = time.perf_counter()
@@ -188,12 +199,27 @@ def bench_batch(args):
         t0 = time.perf_counter()
         res = pool.run(geoms, "6-31g")
         el = time.perf_counter() - t0
-    # per-molecule energies against their own single-molecule runs (sample)
+        print("batch FP64 done", file=sys.stderr, flush=True)
+        # the full form: batched + learned start + FP32 tier below 1e-4
+        pool.run(geoms[:2 * best[0] * best[1]], "6-31g", guess="learned", fp32_below=1e-4)
+        t1 = time.perf_counter()
+        res_full = pool.run(geoms, "6-31g", guess="learned", fp32_below=1e-4)
+        el_full = time.perf_counter() - t1
+        print("batch full form done", file=sys.stderr, flush=True)
+    # per-molecule energies against their own single-molecule runs, and the
+    # plain path (one molecule at a time, FP64, core guess) on a sample
     dmax = 0.0
-    for i in range(0, 512, 64):
+    sample = list(range(0, 512, 8))
+    for i in sample[:4]:   # warm-up
+        Molecule(geoms[i], "6-31g").RHF(doPrint=False, direct=True)
+    t2 = time.perf_counter()
+    for i in sample:
         m = Molecule(geoms[i], "6-31g")
         m.RHF(doPrint=False, direct=True)
         dmax = max(dmax, abs(m.energy.real - res[i]["energy"]))
+    plain_rate = len(sample) / (time.perf_counter() - t2)
+    e64 = np.array([r["energy"] for r in res])
+    efull = np.array([r["energy"] for r in res_full])
     return {
         "metric": METRIC, "value": len(geoms) / el, "unit": "SCF/s", "n_gpus": 1, "steps": 1, "warmup": 0,
         "ms_per_step": 1e3 * el, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
@@ -204,7 +230,30 @@ def bench_batch(args):
         "all_converged": all(r["converged"] for r in res),
         "mean_iterations": float(np.mean([r["iterations"] for r in res])),
         "max_abs_dE_vs_single_runs": dmax,
+        "plain_fp64_one_at_a_time_scf_per_s": plain_rate,
+        "full_form": {"what": "batched + learned start + FP32 tier below 1e-4", "scf_per_s": len(geoms) / el_full,
+                      "all_converged": all(r["converged"] for r in res_full),
+                      "mean_iterations": float(np.mean([r["iterations"] for r in res_full])),
+                      "max_abs_dE_vs_fp64_batch": float(np.max(np.abs(efull - e64))),
+                      "speedup_vs_plain": (len(geoms) / el_full) / plain_rate},
     }
+
+
+def copy_calibration():
+    """A device-to-device copy of 256 MiB (five launches), the reference
+    point for scaling the HBM byte counters of the profiles."""
+    import torch
+    n = 256 << 20
+    src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    return {"metric": "device copy", "bytes_read_per_launch": n, "bytes_written_per_launch": n,
+            "launches": 5, "GB_per_s": 2 * n * 5 / el / 1e9}
 
 
 def main():
@@ -215,7 +264,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
-    ap.add_argument("--workload", choices=("h2o8", "h2o16-tier", "batch512"), default="h2o8",
+    ap.add_argument("--workload", choices=("h2o8", "h2o16-tier", "batch512", "copy-calibration"), default="h2o8",
                     help="h2o8: (H2O)8 cc-pVDZ FP64 (default); h2o16-tier: (H2O)16 cc-pVDZ FP64 vs FP32 tier; "
                          "batch512: 512 small 6-31G clusters")
     args = ap.parse_args()
@@ -227,6 +276,9 @@ def main():
     import numpy as np
     from mmd.workloads import water_cluster
     from mmd import _native
+    if args.workload == "copy-calibration":
+        print(json.dumps(copy_calibration()))
+        return
     if args.workload == "batch512":   # starts its worker processes before any GPU use here
         print(json.dumps(bench_batch(args)))
         return
@@ -246,7 +298,9 @@ def main():
     mols = []
     t0 = time.perf_counter()
     for s in range(args.steps):
-        mols.append(run_scf(geom, basis))
+        # per-class HIP events on the classes' own streams, inside the timed
+        # region (last step)
+        mols.append(run_scf(geom, basis, profile=(not args.no_profile) and s == args.steps - 1))
         print("step %d done" % s, file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
     scf_per_s = args.steps / elapsed
@@ -270,20 +324,20 @@ def main():
     }
 
     if not args.no_profile:
-        pm = run_scf(geom, basis, profile=True)
-        per = roofline_from_stats(pm._profile_stats)
+        stats = [st for m in mols for st in getattr(m, "_profile_stats", [])]
+        per = roofline_from_stats(stats)
         tot_f = sum(v[0] for v in per.values())
         tot_t = sum(v[1] for v in per.values())
         dom = max(per, key=lambda k: per[k][1])
         f, t, n = per[dom][:3]
         achieved = f / (t * 1e-3) / 1e12
         out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                           "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                           "frac": achieved / FP64_PEAK_TFLOPS, "traffic": TRAFFIC.get(dom),
                            "kernel": "eri_fock_class_kernel<%d,%d,%d,%d,double>" % dom,
                            "kernel_avg_ms": t / n, "launches": n,
                            "all_classes_achieved": tot_f / (tot_t * 1e-3) / 1e12,
-                           "all_classes_ms_per_build": tot_t / max(1, len(pm._profile_stats))}
-        nb = max(1, len(pm._profile_stats))
+                           "all_classes_event_ms_per_build": tot_t / max(1, len(stats))}
+        nb = max(1, len(stats))
         out["class_breakdown"] = {
             "(%d%d|%d%d)" % k: {"ms_per_build": v[1] / nb, "tflops": v[0] / (v[1] * 1e-3) / 1e12 if v[1] else 0,
                                 "shell_quartets_per_build": v[3] // nb, "prim_quartets_per_build": v[4] // nb}

@@ -140,8 +140,11 @@ int mmd_boys(mmd_ctx *ctx, int n, const double *m, const double *T, int table, d
  * Per-class statistics of the symmetrised Fock build.  With profiling on,
  * every mmd_fock_delta(raw = 0) records, per shell-quartet class
  * (la lb | lc ld): surviving shell quartets, surviving primitive shell
- * quartets and the class kernel's device time (HIP events on the context's
- * stream).  No counterpart in the reference (instrumentation).
+ * quartets and the class kernel's device time (HIP events on the stream the
+ * class runs on).  on = 1 keeps the classes concurrent (times include the
+ * sharing of the GPU), on = 2 runs them one after the other (each timed
+ * alone), on = 0 turns profiling off.  No counterpart in the reference
+ * (instrumentation).
  * mmd_fock_stats fills up to 21 entries (lclass: 4 per class) and returns
  * the number of classes.
  */
@@ -162,6 +165,14 @@ int mmd_last_fock_ms(mmd_ctx *ctx, double *ms);
  */
 int mmd_set_precision(mmd_ctx *ctx, double fp32_below);
 int mmd_fock_tiers(mmd_ctx *ctx, int64_t *nquart, int64_t *nquart_fp32);
+
+/*
+ * Number of streams (1..4, default 4) the shell-quartet classes of a Fock
+ * build are spread over.  No counterpart in the reference: one stream suits
+ * callers that run many contexts side by side (batches of small molecules),
+ * where the other contexts supply the concurrency.
+ */
+int mmd_set_streams(mmd_ctx *ctx, int n);
 
 /*
  * Multilayer perceptron of the learned initial density (no counterpart in

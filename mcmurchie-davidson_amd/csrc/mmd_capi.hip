@@ -82,6 +82,7 @@ struct mmd_ctx {
     static constexpr int NSIDE = 3;
     hipStream_t side[NSIDE] = {};
     hipEvent_t evfork = nullptr, evjoin[NSIDE] = {};
+    int nstreams = 1 + NSIDE;           // streams the Fock classes are spread over
     double last_fock_ms = 0.0;
     int64_t last_nsurv = 0, last_nsurv_fp32 = 0;   // shell quartets of the last symmetrised build
     double fp32_below = 0.0;                        // FP32 tier: quartets with bound below this (0: off)
@@ -129,11 +130,12 @@ struct mmd_ctx {
     DevBuf<double> d_D, d_Dim, d_absD, d_Dsh, d_G, d_Gout;
 
     // per-class statistics of the last symmetrised Fock build
-    bool profiling = false;
+    int profiling = 0;                  // 0 off; 1 events, classes concurrent; 2 events, classes serial
     hipEvent_t evc[2 * NQCLASS] = {};
     DevBuf<unsigned long long> d_nprim;   // [0,21): primitive quartets, [21,42): shell quartets
     DevBuf<int> d_rowctr;
     size_t max_lds = 65536;
+    bool last_row_kernel[NQCLASS] = {};
     int64_t st_nquart[NQCLASS] = {0};
     int64_t st_nprim[NQCLASS] = {0};
     double st_ms[NQCLASS] = {0};
@@ -391,8 +393,10 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol)
     }
     // Classes run concurrently on the main and side streams (longest first,
     // each to the least loaded stream), so that the small classes fill the
-    // GPU beside the large ones.  With profiling on they run one after the
-    // other on the main stream, so each class's events time it alone.
+    // GPU beside the large ones.  Profiling mode 1 brackets every class with
+    // events on its own stream (times include sharing the GPU with the other
+    // classes); mode 2 runs the classes one after the other on the main
+    // stream, so that each class's events time it alone.
     double pc_npp[NPAIRCLASS] = {0};
     for (int pc = 0; pc < NPAIRCLASS; ++pc)
         for (int P = c->pc_off[pc]; P < c->pc_off[pc + 1]; ++P) pc_npp[pc] += c->pairs[P].npp;
@@ -409,7 +413,7 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol)
         order[nord++] = qc;
     }
     std::sort(order, order + nord, [&](int x, int y) { return cost[x] > cost[y]; });
-    const int nstream = c->profiling ? 1 : 1 + mmd_ctx::NSIDE;
+    const int nstream = c->profiling == 2 ? 1 : c->nstreams;
     hipStream_t streams[1 + mmd_ctx::NSIDE] = {c->st};
     for (int i = 0; i < mmd_ctx::NSIDE; ++i) streams[1 + i] = c->side[i];
     double load[1 + mmd_ctx::NSIDE] = {0};
@@ -482,21 +486,27 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol)
         HIP_TRY(hipEventRecord(c->evjoin[i - 1], streams[i]));
         HIP_TRY(hipStreamWaitEvent(c->st, c->evjoin[i - 1], 0));
     }
-    if (c->profiling) {
-        int h[NQCLASS];
-        unsigned long long hp[3 * NQCLASS];
-        HIP_TRY(hipMemcpyAsync(h, c->d_count.p, sizeof(int) * NQCLASS, hipMemcpyDeviceToHost, c->st));
-        HIP_TRY(hipMemcpyAsync(hp, c->d_nprim.p, sizeof(hp), hipMemcpyDeviceToHost, c->st));
-        HIP_TRY(hipStreamSynchronize(c->st));
-        for (int qc = 0; qc < NQCLASS; ++qc) {
-            c->st_nquart[qc] = c->qc_total[qc]
-                                   ? (row_kernel[qc] ? (int64_t)(hp[NQCLASS + qc] + hp[2 * NQCLASS + qc]) : h[qc])
-                                   : 0;
-            c->st_nprim[qc] = c->qc_total[qc] ? (int64_t)hp[qc] : 0;
-            float ms = 0.f;
-            if (c->qc_total[qc]) HIP_TRY(hipEventElapsedTime(&ms, c->evc[2 * qc], c->evc[2 * qc + 1]));
-            c->st_ms[qc] = ms;
-        }
+    for (int qc = 0; qc < NQCLASS; ++qc) c->last_row_kernel[qc] = row_kernel[qc];
+    return MMD_OK;
+}
+
+// Per-class statistics and event times of the last build (profiling on).
+// Called after the build's end event, so the readback is not timed.
+int collect_profile(mmd_ctx* c)
+{
+    int h[NQCLASS];
+    unsigned long long hp[3 * NQCLASS];
+    HIP_TRY(hipMemcpyAsync(h, c->d_count.p, sizeof(int) * NQCLASS, hipMemcpyDeviceToHost, c->st));
+    HIP_TRY(hipMemcpyAsync(hp, c->d_nprim.p, sizeof(hp), hipMemcpyDeviceToHost, c->st));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    for (int qc = 0; qc < NQCLASS; ++qc) {
+        c->st_nquart[qc] = c->qc_total[qc]
+                               ? (c->last_row_kernel[qc] ? (int64_t)(hp[NQCLASS + qc] + hp[2 * NQCLASS + qc]) : h[qc])
+                               : 0;
+        c->st_nprim[qc] = c->qc_total[qc] ? (int64_t)hp[qc] : 0;
+        float ms = 0.f;
+        if (c->qc_total[qc]) HIP_TRY(hipEventElapsedTime(&ms, c->evc[2 * qc], c->evc[2 * qc + 1]));
+        c->st_ms[qc] = ms;
     }
     return MMD_OK;
 }
@@ -849,6 +859,7 @@ int mmd_fock_delta(mmd_ctx* ctx, const double* dP_re, const double* dP_im, doubl
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->last_fock_ms = ms;
+    if (ctx->profiling && !raw) return collect_profile(ctx);
     return MMD_OK;
 }
 
@@ -910,7 +921,8 @@ int mmd_set_profiling(mmd_ctx* ctx, int on)
     if (rc) return rc;
     if (on && !ctx->evc[0])
         for (int i = 0; i < 2 * NQCLASS; ++i) HIP_TRY(hipEventCreate(&ctx->evc[i]));
-    ctx->profiling = on != 0;
+    if (on < 0 || on > 2) return fail(MMD_ERR_ARG, "profiling mode must be 0, 1 or 2");
+    ctx->profiling = on;
     return MMD_OK;
 }
 
@@ -931,6 +943,14 @@ int mmd_set_precision(mmd_ctx* ctx, double fp32_below)
     if (!ctx) return fail(MMD_ERR_ARG, "null context");
     if (!(fp32_below >= 0.0) || fp32_below > 1.0) return fail(MMD_ERR_ARG, "fp32_below must be in [0, 1]");
     ctx->fp32_below = fp32_below;
+    return MMD_OK;
+}
+
+int mmd_set_streams(mmd_ctx* ctx, int n)
+{
+    if (!ctx) return fail(MMD_ERR_ARG, "null context");
+    if (n < 1 || n > 1 + mmd_ctx::NSIDE) return fail(MMD_ERR_ARG, "streams must be in [1, %d]", 1 + mmd_ctx::NSIDE);
+    ctx->nstreams = n;
     return MMD_OK;
 }
 

@@ -415,12 +415,19 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
     const int upr = (R.nk + R.seg - 1) / R.seg;          // units per row
     const int nunits = R.nb * upr;
     const size_t ns = (size_t)R.nshell;
+    unsigned long long my_nprim = 0;   // primitive quartets of this lane (statistics)
     for (;;) {
         __syncthreads();
         if (threadIdx.x == 0) misc[0] = atomicAdd(R.row_counter, 1);
         __syncthreads();
         const int u = misc[0];
-        if (u >= nunits) break;
+        if (u >= nunits) {
+            if (R.stat_nprim) {   // one atomic per wavefront
+                for (int off = 32; off > 0; off >>= 1) my_nprim += __shfl_xor(my_nprim, off);
+                if (lane == 0 && my_nprim) atomicAdd(R.stat_nprim, my_nprim);
+            }
+            break;
+        }
         // triangular classes hand out their long rows first
         const int P = R.tri ? (R.nb - 1 - u / upr) : u / upr;
         const int k0 = (u % upr) * R.seg;
@@ -488,8 +495,7 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
                     if (jq0 >= ket.npp) continue;
                     const int jq1 = min(ket.npp, jq0 + PSLICE);
                     const int ip0 = bs * PSLICE, ip1 = min(bra.npp, ip0 + PSLICE);
-                    if (R.stat_nprim && chunk == 0)
-                        atomicAdd(R.stat_nprim, (unsigned long long)(ip1 - ip0) * (unsigned long long)(jq1 - jq0));
+                    if (chunk == 0) my_nprim += (unsigned long long)(ip1 - ip0) * (unsigned long long)(jq1 - jq0);
                     static_switch<0, S::NCHUNK>(chunk, [&](auto ic) {
                         constexpr int C = decltype(ic)::value;
                         constexpr int A0 = C * S::CH, A1 = (C + 1) * S::CH;

@@ -72,6 +72,8 @@ lib.mmd_last_fock_ms.restype = _c_int
 lib.mmd_last_fock_ms.argtypes = [_ctx_p, ctypes.POINTER(_c_double)]
 lib.mmd_set_precision.restype = _c_int
 lib.mmd_set_precision.argtypes = [_ctx_p, _c_double]
+lib.mmd_set_streams.restype = _c_int
+lib.mmd_set_streams.argtypes = [_ctx_p, _c_int]
 lib.mmd_fock_tiers.restype = _c_int
 lib.mmd_fock_tiers.argtypes = [_ctx_p, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64)]
 lib.mmd_mlp_create.restype = _c_int
@@ -96,6 +98,7 @@ EXPORTED = [
     "mmd_set_basis", "mmd_set_atoms", "mmd_one_electron", "mmd_schwarz", "mmd_set_schwarz",
     "mmd_eri_tensor", "mmd_eri_list", "mmd_fock_delta", "mmd_jk_tensor", "mmd_boys",
     "mmd_last_fock_ms", "mmd_set_profiling", "mmd_fock_stats", "mmd_set_precision", "mmd_fock_tiers",
+    "mmd_set_streams",
     "mmd_mlp_create", "mmd_mlp_destroy", "mmd_mlp_nparam", "mmd_mlp_set_params", "mmd_mlp_get_params",
     "mmd_mlp_forward", "mmd_mlp_loss_grad", "mmd_mlp_train",
 ]
@@ -253,8 +256,10 @@ class Context:
         check(lib.mmd_boys(self._h, m.size, _ptr(m), _ptr(T), 1 if table else 0, _ptr(out)))
         return out
 
-    def set_profiling(self, on=True):
-        check(lib.mmd_set_profiling(self._h, 1 if on else 0))
+    def set_profiling(self, on=True, serial=False):
+        """Per-class statistics and event times of every symmetrised build;
+        serial=True runs the classes one after the other (isolated times)."""
+        check(lib.mmd_set_profiling(self._h, (2 if serial else 1) if on else 0))
 
     def fock_stats(self):
         """Per-class (l_a, l_b, l_c, l_d), surviving shell quartets, surviving
@@ -277,6 +282,10 @@ class Context:
         """FP32 tier: shell quartets with screening bound below fp32_below
         get FP32 integrals (FP64 digestion); 0 keeps everything FP64."""
         check(lib.mmd_set_precision(self._h, float(fp32_below)))
+
+    def set_streams(self, n):
+        """Streams (1..4) the Fock-build classes are spread over."""
+        check(lib.mmd_set_streams(self._h, int(n)))
 
     def fock_tiers(self):
         """(surviving shell quartets, of which FP32) of the last symmetrised build."""

@@ -33,13 +33,15 @@ def _worker_ctx(local, device):
     ctx = getattr(local, "ctx", None)
     if ctx is None:
         ctx = _native.Context(device)
+        ctx.set_streams(1)   # the other workers' contexts supply the concurrency
         local.ctx = ctx
     return ctx
 
 
 def _run_one(geom, basis, kwargs, local, device, keep_density):
     mol = Molecule(geom, basis)
-    mol.attach_context(_worker_ctx(local, device))
+    ctx = mol.attach_context(_worker_ctx(local, device))
+    ctx.set_precision(kwargs.get("fp32_below", 0.0))
     t0 = time.perf_counter()
     mol._rhf(False, kwargs.get("DIIS", True), kwargs.get("direct", True), kwargs.get("conver", 1e-8),
              kwargs.get("acc2e", 1e-12), kwargs.get("guess", "core"))
@@ -53,7 +55,8 @@ def _run_one(geom, basis, kwargs, local, device, keep_density):
 
 def run_batch(geometries, basis="6-31g", workers=8, device=0, keep_density=False, executor=None, **rhf_kwargs):
     """RHF energies of all geometries.  rhf_kwargs: DIIS, direct (default
-    True), conver, acc2e, guess.  Returns one dict per geometry, in order:
+    True), conver, acc2e, guess, fp32_below (FP32 tier of the Fock build,
+    0 = all FP64).  Returns one dict per geometry, in order:
     energy, iterations, converged, nbasis, seconds (and the converged density
     P with keep_density).  ``executor``: a thread pool to reuse (its
     threads keep their GPU contexts between calls)."""
@@ -83,6 +86,9 @@ class BatchPool:
         pkg = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
         env = dict(os.environ)
         env["PYTHONPATH"] = pkg + (os.pathsep + env["PYTHONPATH"] if env.get("PYTHONPATH") else "")
+        # one hardware queue per worker thread (its context's stream), so that
+        # the pool does not oversubscribe the GPU's queues
+        env["GPU_MAX_HW_QUEUES"] = str(min(threads, 4))
         self.procs, self.threads = procs, threads
         self._ps = [subprocess.Popen([sys.executable, "-m", "mmd.batch", "--serve", str(threads), str(device)],
                                      stdin=subprocess.PIPE, stdout=subprocess.PIPE, env=env) for _ in range(procs)]

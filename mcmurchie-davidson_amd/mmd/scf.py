@@ -14,16 +14,30 @@ import scipy.linalg
 
 from mmd.integrals.fock import formPT
 
-try:  # small dense algebra (N <= a few hundred) is fastest on one BLAS thread
-    from threadpoolctl import threadpool_limits as _threadpool_limits
+try:  # BLAS thread counts are chosen per operation (see SCF.RHF)
+    from threadpoolctl import ThreadpoolController as _Controller
 except ImportError:  # pragma: no cover
-    _threadpool_limits = None
+    _Controller = None
+_ctl = None
 
 
 def _blas_threads(n):
-    if _threadpool_limits is None:
+    """Context that limits the BLAS/LAPACK thread pool to n threads (None:
+    leave it as it is)."""
+    global _ctl
+    if _Controller is None or n is None:
         return contextlib.nullcontext()
-    return _threadpool_limits(limits=n, user_api="blas")
+    if _ctl is None:
+        _ctl = _Controller()
+    return _ctl.limit(limits=n, user_api="blas")
+
+
+def _thread_plan(N):
+    """(threads for the matrix products, threads for the eigensolver): the
+    products of N >= 128 use 4 threads; the divide-and-conquer eigensolver
+    is no faster on more threads at N <= 400 (and slower at N ~ 200), so it
+    runs on one."""
+    return (1 if N < 128 else 4), 1
 
 
 def _re(A):
@@ -49,8 +63,12 @@ class SCF(object):
         per water monomer by the trained perceptron (mmd.learned)."""
         if guess not in ('core', 'sad', 'learned'):
             raise ValueError("guess must be 'core', 'sad' or 'learned'")
-        with _blas_threads(1 if self.nbasis < 600 else 4):
-            return self._rhf(doPrint, DIIS, direct, conver, acc2e, guess)
+        gemm_threads, self._eigh_threads = _thread_plan(self.nbasis)
+        try:
+            with _blas_threads(gemm_threads):
+                return self._rhf(doPrint, DIIS, direct, conver, acc2e, guess)
+        finally:
+            self._eigh_threads = None
 
     def _rhf(self, doPrint, DIIS, direct, conver, acc2e, guess='core'):
         self.is_converged = False
@@ -94,7 +112,8 @@ class SCF(object):
             if not DIIS or step == 0:
                 self.orthoFock()
 
-            E, self.CO = scipy.linalg.eigh(_re(self.FO), driver="evd")
+            with _blas_threads(getattr(self, "_eigh_threads", None)):
+                E, self.CO = scipy.linalg.eigh(_re(self.FO), driver="evd")
             C = np.dot(self.X, self.CO)
             self.C = C
             self.MO = E

@@ -19,6 +19,11 @@ def _fact2(n):
     return r
 
 
+# normalised (norm, coefs) per (shell, exponents, coefficients): the same
+# contracted function recurs on every atom of an element
+_NORMALIZED = {}
+
+
 class Basis(object):
     """One Cartesian component of a contracted Gaussian (cython/basis.pxi:6-85).
 
@@ -33,10 +38,19 @@ class Basis(object):
         self._origin = np.array([float(origin[i]) for i in range(3)], dtype=np.float64)
         self._shell = np.array([int(shell[i]) for i in range(3)], dtype=np.int64)
         self._num_exps = int(num_exps)
-        self._exps = np.array([float(exps[i]) for i in range(self._num_exps)], dtype=np.float64)
-        self._coefs = np.array([float(coefs[i]) for i in range(self._num_exps)], dtype=np.float64)
-        self._norm = np.zeros(self._num_exps, dtype=np.float64)
-        self.normalize()
+        self._exps = np.array(exps[:self._num_exps], dtype=np.float64)
+        self._coefs = np.array(coefs[:self._num_exps], dtype=np.float64)
+        if self._exps.shape != (self._num_exps,) or self._coefs.shape != (self._num_exps,):
+            raise ValueError("exps and coefs need num_exps entries")
+        key = (tuple(self._shell.tolist()), self._exps.tobytes(), self._coefs.tobytes())
+        cached = _NORMALIZED.get(key)
+        if cached is None:
+            self._norm = np.zeros(self._num_exps, dtype=np.float64)
+            self.normalize()
+            _NORMALIZED[key] = (self._norm.copy(), self._coefs.copy())
+        else:
+            self._norm = cached[0].copy()
+            self._coefs = cached[1].copy()
 
     origin = property(lambda self: self._origin)
     shell = property(lambda self: self._shell)

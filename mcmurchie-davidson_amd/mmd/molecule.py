@@ -44,6 +44,9 @@ _SHELLS = {
 }
 
 
+_BASIS_FILES = {}
+
+
 class Atom(object):
     """An atom: nuclear charge, mass (a.u.) and position (bohr)."""
 
@@ -141,7 +144,14 @@ class Molecule(SCF):
 
     def getBasis(self, filename):
         """Parse a Gaussian-94 basis file: {Z: [(momentum, [(exp, coef), ...]), ...]}.
-        SP shells become an S and a P entry (mmd/molecule.py:135-189)."""
+        SP shells become an S and a P entry (mmd/molecule.py:135-189).  A file
+        is parsed once per process; the result is shared read-only."""
+        key = os.path.abspath(filename)
+        if key not in _BASIS_FILES:
+            _BASIS_FILES[key] = self._parse_basis(filename)
+        return _BASIS_FILES[key]
+
+    def _parse_basis(self, filename):
         basis = {}
         with open(filename, 'r') as fh:
             data = fh.read().split('****')

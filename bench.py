@@ -58,15 +58,22 @@ def run_scf(geom, basis, profile=False, serial=False):
     from mmd.molecule import Molecule
     mol = Molecule(geom, basis)
     if profile:
-        mol.native.set_profiling(True, serial=serial)
+        ctx = mol.native
+        ctx.set_profiling(True, serial=serial)
         mol._profile_stats = []
-        orig = mol.native.fock_delta
+        orig_delta, orig_step = ctx.fock_delta, ctx.scf_step
 
-        def wrapped(*a, **k):
-            r = orig(*a, **k)
-            mol._profile_stats.append(mol.native.fock_stats())
+        def delta(*a, **k):
+            r = orig_delta(*a, **k)
+            mol._profile_stats.append(ctx.fock_stats())
             return r
-        mol.native.fock_delta = wrapped
+
+        def step(tol, build, diis, FO):   # the GPU-resident loop builds inside scf_step
+            r = orig_step(tol, build, diis, FO)
+            if build:
+                mol._profile_stats.append(ctx.fock_stats())
+            return r
+        ctx.fock_delta, ctx.scf_step = delta, step
     mol.RHF(doPrint=False, direct=True, conver=1e-8, acc2e=1e-12)
     return mol
 
@@ -132,15 +139,22 @@ def bench_tier(args):
 
     def run(fp32_below):
         mol = Molecule(geom, "cc-pvdz")
-        mol.native.set_precision(fp32_below)
+        ctx = mol.native
+        ctx.set_precision(fp32_below)
         tiers = []
-        orig = mol.native.fock_delta
+        orig_delta, orig_step = ctx.fock_delta, ctx.scf_step
 
-        def wrapped(*a, **k):
-            r = orig(*a, **k)
-            tiers.append(mol.native.fock_tiers())
+        def delta(*a, **k):
+            r = orig_delta(*a, **k)
+            tiers.append(ctx.fock_tiers())
             return r
-        mol.native.fock_delta = wrapped
+
+        def step(tol, build, diis, FO):
+            r = orig_step(tol, build, diis, FO)
+            if build:
+                tiers.append(ctx.fock_tiers())
+            return r
+        ctx.fock_delta, ctx.scf_step = delta, step
         mol.RHF(doPrint=False, direct=True, conver=1e-8, acc2e=1e-12)
         mol._tiers = tiers
         return mol

@@ -175,6 +175,28 @@ int mmd_fock_tiers(mmd_ctx *ctx, int64_t *nquart, int64_t *nquart_fp32);
 int mmd_set_streams(mmd_ctx *ctx, int n);
 
 /*
+ * GPU-resident state of the direct RHF loop (mmd/scf.py:10-80 of the
+ * reference; the caller runs the loop and the eigensolver).  No counterpart
+ * in the reference: H, S, X = S^-1/2, F and the density stay in device
+ * memory between Fock builds, with the DIIS window (depth 8, error
+ * X (FPS - SPF) X, mmd/scf.py:146-180).  Host pointers, N*N row-major.
+ *   scf_begin   : H (core), S, X, the current F and density P (= P_old)
+ *   scf_step    : build = 1: F += G(P - P_old) (the symmetrised
+ *                 mmd_fock_delta build), P_old = P and, with diis, the DIIS
+ *                 extrapolation; then FO = X^T F X (extrapolated F with
+ *                 diis) is copied to the host
+ *   scf_density : from the eigenvectors C' of FO: C = X C',
+ *                 P = C_occ C_occ^T; stats = {sum (H + F)_pq P_qp,
+ *                 ||P - P_old||_F}
+ *   scf_get     : copies F, P, P_old, G (last build) to the host (NULL skips)
+ */
+int mmd_scf_begin(mmd_ctx *ctx, int nocc, const double *H, const double *S, const double *X, const double *F,
+                  const double *P);
+int mmd_scf_step(mmd_ctx *ctx, double tol, int build, int diis, double *FO);
+int mmd_scf_density(mmd_ctx *ctx, const double *CO, double *stats);
+int mmd_scf_get(mmd_ctx *ctx, double *F, double *P, double *P_old, double *G);
+
+/*
  * Multilayer perceptron of the learned initial density (no counterpart in
  * the reference): tanh hidden layers, linear output layer, FP32 HIP kernels.
  *   dims            : nlayer + 1 widths (input, hidden..., output)

@@ -14,8 +14,10 @@
 #include <vector>
 
 #include "../../include/mmd.h"
+#include <rocblas/rocblas.h>
 #include "mmd_device.h"
 #include "mmd_kernels.h"
+#include "mmd_scf.h"
 #include "mmd_types.h"
 
 using namespace mmd;
@@ -137,6 +139,14 @@ struct mmd_ctx {
     size_t max_lds = 65536;
     bool last_row_kernel[NQCLASS] = {};
     int64_t st_nquart[NQCLASS] = {0};
+
+    // GPU-resident SCF loop (mmd_scf_*): matrices, DIIS window, BLAS handle
+    rocblas_handle blas = nullptr;
+    int scf_n = 0, scf_nocc = 0;
+    DevBuf<double> s_H, s_S, s_X, s_F, s_P, s_Pold, s_G, s_dP, s_T1, s_T2, s_C, s_stats, s_dots;
+    DevBuf<double> s_Fset, s_Eset;   // DIIS window: 8 slots of N*N each
+    std::vector<int> s_order;        // slots of the window, oldest first
+    std::vector<double> s_B;         // 8 x 8 error overlaps, window order
     int64_t st_nprim[NQCLASS] = {0};
     double st_ms[NQCLASS] = {0};
 };
@@ -624,6 +634,7 @@ void mmd_ctx_destroy(mmd_ctx* ctx)
         if (ctx->side[i]) (void)hipStreamDestroy(ctx->side[i]);
     }
     if (ctx->evfork) (void)hipEventDestroy(ctx->evfork);
+    if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
     hipStream_t st = ctx->st;
     delete ctx;  // frees device buffers
     if (st) (void)hipStreamDestroy(st);
@@ -959,6 +970,209 @@ int mmd_fock_tiers(mmd_ctx* ctx, int64_t* nquart, int64_t* nquart_fp32)
     if (!ctx) return fail(MMD_ERR_ARG, "null context");
     if (nquart) *nquart = ctx->last_nsurv;
     if (nquart_fp32) *nquart_fp32 = ctx->last_nsurv_fp32;
+    return MMD_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Row-major C (m x n) = op(A) op(B), op = transpose where tA / tB; the
+// column-major BLAS computes C^T = op(B)^T op(A)^T.
+int gemm_rm(mmd_ctx* c, bool tA, bool tB, int m, int n, int k, const double* A, int lda, const double* B, int ldb,
+            double* C, int ldc)
+{
+    const double one = 1.0, zero = 0.0;
+    const rocblas_status s = rocblas_dgemm(c->blas, tB ? rocblas_operation_transpose : rocblas_operation_none,
+                                           tA ? rocblas_operation_transpose : rocblas_operation_none, n, m, k, &one,
+                                           B, ldb, A, lda, &zero, C, ldc);
+    if (s != rocblas_status_success) return fail(MMD_ERR_DEVICE, "rocblas_dgemm: status %d", (int)s);
+    return MMD_OK;
+}
+
+// Solve the (n x n) system A x = b in place (Gaussian elimination with
+// partial pivoting); false if singular.
+bool solve_small(int n, std::vector<double>& A, std::vector<double>& b)
+{
+    for (int k = 0; k < n; ++k) {
+        int piv = k;
+        for (int i = k + 1; i < n; ++i)
+            if (fabs(A[i * n + k]) > fabs(A[piv * n + k])) piv = i;
+        if (A[piv * n + k] == 0.0) return false;
+        if (piv != k) {
+            for (int j = 0; j < n; ++j) std::swap(A[k * n + j], A[piv * n + j]);
+            std::swap(b[k], b[piv]);
+        }
+        for (int i = k + 1; i < n; ++i) {
+            const double f = A[i * n + k] / A[k * n + k];
+            for (int j = k; j < n; ++j) A[i * n + j] -= f * A[k * n + j];
+            b[i] -= f * b[k];
+        }
+    }
+    for (int k = n - 1; k >= 0; --k) {
+        double v = b[k];
+        for (int j = k + 1; j < n; ++j) v -= A[k * n + j] * b[j];
+        b[k] = v / A[k * n + k];
+    }
+    return true;
+}
+
+int scf_check(mmd_ctx* c)
+{
+    if (c->scf_n == 0 || c->scf_n != c->nbf) return fail(MMD_ERR_STATE, "mmd_scf_begin must be called first");
+    return MMD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mmd_scf_begin(mmd_ctx* ctx, int nocc, const double* H, const double* S, const double* X, const double* F,
+                  const double* P)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    const int N = ctx->nbf;
+    if (N == 0) return fail(MMD_ERR_STATE, "no basis set");
+    if (nocc < 1 || nocc > N || !H || !S || !X || !F || !P) return fail(MMD_ERR_ARG, "invalid SCF arguments");
+    if (!ctx->blas) {
+        if (rocblas_create_handle(&ctx->blas) != rocblas_status_success)
+            return fail(MMD_ERR_DEVICE, "rocblas_create_handle failed");
+    }
+    if (rocblas_set_stream(ctx->blas, ctx->st) != rocblas_status_success)
+        return fail(MMD_ERR_DEVICE, "rocblas_set_stream failed");
+    const size_t nn = (size_t)N * N;
+    for (DevBuf<double>* b : {&ctx->s_H, &ctx->s_S, &ctx->s_X, &ctx->s_F, &ctx->s_P, &ctx->s_Pold, &ctx->s_G,
+                              &ctx->s_dP, &ctx->s_T1, &ctx->s_T2, &ctx->s_C})
+        HIP_TRY(b->reserve(nn));
+    HIP_TRY(ctx->s_Fset.reserve(8 * nn));
+    HIP_TRY(ctx->s_Eset.reserve(8 * nn));
+    HIP_TRY(ctx->s_stats.reserve(2));
+    HIP_TRY(ctx->s_dots.reserve(8));
+    const size_t by = sizeof(double) * nn;
+    HIP_TRY(hipMemcpyAsync(ctx->s_H.p, H, by, hipMemcpyHostToDevice, ctx->st));
+    HIP_TRY(hipMemcpyAsync(ctx->s_S.p, S, by, hipMemcpyHostToDevice, ctx->st));
+    HIP_TRY(hipMemcpyAsync(ctx->s_X.p, X, by, hipMemcpyHostToDevice, ctx->st));
+    HIP_TRY(hipMemcpyAsync(ctx->s_F.p, F, by, hipMemcpyHostToDevice, ctx->st));
+    HIP_TRY(hipMemcpyAsync(ctx->s_P.p, P, by, hipMemcpyHostToDevice, ctx->st));
+    HIP_TRY(hipMemcpyAsync(ctx->s_Pold.p, P, by, hipMemcpyHostToDevice, ctx->st));
+    HIP_TRY(launch_fill(nn, ctx->s_G.p, 0.0, ctx->st));
+    HIP_TRY(hipStreamSynchronize(ctx->st));
+    ctx->scf_n = N;
+    ctx->scf_nocc = nocc;
+    ctx->s_order.clear();
+    ctx->s_B.assign(64, 0.0);
+    return MMD_OK;
+}
+
+int mmd_scf_step(mmd_ctx* ctx, double tol, int build, int diis, double* FO)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if ((rc = scf_check(ctx))) return rc;
+    if (!FO) return fail(MMD_ERR_ARG, "null pointer");
+    const int N = ctx->nbf;
+    const size_t nn = (size_t)N * N;
+    hipStream_t st = ctx->st;
+    const double* Fuse = ctx->s_F.p;
+    if (build) {
+        // F <- F + G(P - P_old); P_old <- P   (mmd/scf.py:84-94)
+        HIP_TRY(launch_scf_axpby(nn, 1.0, ctx->s_P.p, -1.0, ctx->s_Pold.p, ctx->s_dP.p, st));
+        rc = mmd_fock_delta(ctx, ctx->s_dP.p, nullptr, tol, 0, ctx->s_G.p, nullptr, 1, nullptr);
+        if (rc) return rc;
+        HIP_TRY(launch_scf_axpby(nn, 1.0, ctx->s_F.p, 1.0, ctx->s_G.p, ctx->s_F.p, st));
+        HIP_TRY(hipMemcpyAsync(ctx->s_Pold.p, ctx->s_P.p, sizeof(double) * nn, hipMemcpyDeviceToDevice, st));
+        if (diis) {
+            // error X (FPS - SPF) X of the new F, into the window (mmd/scf.py:146-180)
+            int slot;
+            std::vector<int>& ord = ctx->s_order;
+            std::vector<double>& B = ctx->s_B;
+            if ((int)ord.size() == 8) {
+                slot = ord.front();
+                ord.erase(ord.begin());
+                for (int i = 0; i < 7; ++i)
+                    for (int j = 0; j < 7; ++j) B[i * 8 + j] = B[(i + 1) * 8 + (j + 1)];
+            } else {
+                slot = (int)ord.size();
+            }
+            double* Es = ctx->s_Eset.p + (size_t)slot * nn;
+            double* Fs = ctx->s_Fset.p + (size_t)slot * nn;
+            if ((rc = gemm_rm(ctx, false, false, N, N, N, ctx->s_P.p, N, ctx->s_S.p, N, ctx->s_T1.p, N))) return rc;
+            if ((rc = gemm_rm(ctx, false, false, N, N, N, ctx->s_F.p, N, ctx->s_T1.p, N, ctx->s_T2.p, N))) return rc;
+            HIP_TRY(launch_scf_antisym(N, ctx->s_T2.p, ctx->s_T1.p, st));
+            if ((rc = gemm_rm(ctx, false, false, N, N, N, ctx->s_X.p, N, ctx->s_T1.p, N, ctx->s_T2.p, N))) return rc;
+            if ((rc = gemm_rm(ctx, false, false, N, N, N, ctx->s_T2.p, N, ctx->s_X.p, N, Es, N))) return rc;
+            HIP_TRY(hipMemcpyAsync(Fs, ctx->s_F.p, sizeof(double) * nn, hipMemcpyDeviceToDevice, st));
+            ord.push_back(slot);
+            const int n = (int)ord.size();
+            const double* E[8];
+            const double* Fk[8];
+            for (int k = 0; k < n; ++k) {
+                E[k] = ctx->s_Eset.p + (size_t)ord[k] * nn;
+                Fk[k] = ctx->s_Fset.p + (size_t)ord[k] * nn;
+            }
+            double dots[8];
+            HIP_TRY(launch_scf_dots(n, nn, E, Es, ctx->s_dots.p, st));
+            HIP_TRY(hipMemcpyAsync(dots, ctx->s_dots.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            for (int k = 0; k < n; ++k) B[(n - 1) * 8 + k] = B[k * 8 + (n - 1)] = dots[k];
+            // [B -1; -1 0] w = [0 .. 0 -1]
+            const int m = n + 1;
+            std::vector<double> A((size_t)m * m, 0.0), w(m, 0.0);
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < n; ++j) A[i * m + j] = B[i * 8 + j];
+            for (int i = 0; i < n; ++i) A[i * m + n] = A[n * m + i] = -1.0;
+            w[n] = -1.0;
+            if (!solve_small(m, A, w)) return fail(MMD_ERR_STATE, "singular DIIS matrix");
+            double sw = 0.0;
+            for (int k = 0; k < n; ++k) sw += w[k];
+            if (!(fabs(sw - 1.0) <= 1e-8 + 1e-5)) return fail(MMD_ERR_STATE, "DIIS weights do not sum to one");
+            HIP_TRY(launch_scf_combine(nn, n, Fk, w.data(), ctx->s_C.p, st));
+            Fuse = ctx->s_C.p;
+        }
+    }
+    // FO = X^T F X
+    if ((rc = gemm_rm(ctx, true, false, N, N, N, ctx->s_X.p, N, Fuse, N, ctx->s_T1.p, N))) return rc;
+    if ((rc = gemm_rm(ctx, false, false, N, N, N, ctx->s_T1.p, N, ctx->s_X.p, N, ctx->s_T2.p, N))) return rc;
+    HIP_TRY(hipMemcpyAsync(FO, ctx->s_T2.p, sizeof(double) * nn, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return MMD_OK;
+}
+
+int mmd_scf_density(mmd_ctx* ctx, const double* CO, double* stats)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if ((rc = scf_check(ctx))) return rc;
+    if (!CO || !stats) return fail(MMD_ERR_ARG, "null pointer");
+    const int N = ctx->nbf;
+    const size_t nn = (size_t)N * N;
+    hipStream_t st = ctx->st;
+    // C = X C'; P = C_occ C_occ^T; E_el = sum (H + F)_pq P_qp; ||P - P_old||^2
+    HIP_TRY(hipMemcpyAsync(ctx->s_T1.p, CO, sizeof(double) * nn, hipMemcpyHostToDevice, st));
+    if ((rc = gemm_rm(ctx, false, false, N, N, N, ctx->s_X.p, N, ctx->s_T1.p, N, ctx->s_T2.p, N))) return rc;
+    if ((rc = gemm_rm(ctx, false, true, N, N, ctx->scf_nocc, ctx->s_T2.p, N, ctx->s_T2.p, N, ctx->s_P.p, N)))
+        return rc;
+    HIP_TRY(launch_scf_energy(N, ctx->s_H.p, ctx->s_F.p, ctx->s_P.p, ctx->s_Pold.p, ctx->s_stats.p, st));
+    double h[2];
+    HIP_TRY(hipMemcpyAsync(h, ctx->s_stats.p, sizeof(h), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    stats[0] = h[0];
+    stats[1] = sqrt(h[1]);
+    return MMD_OK;
+}
+
+int mmd_scf_get(mmd_ctx* ctx, double* F, double* P, double* P_old, double* G)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if ((rc = scf_check(ctx))) return rc;
+    const size_t by = sizeof(double) * (size_t)ctx->nbf * ctx->nbf;
+    if (F) HIP_TRY(hipMemcpyAsync(F, ctx->s_F.p, by, hipMemcpyDeviceToHost, ctx->st));
+    if (P) HIP_TRY(hipMemcpyAsync(P, ctx->s_P.p, by, hipMemcpyDeviceToHost, ctx->st));
+    if (P_old) HIP_TRY(hipMemcpyAsync(P_old, ctx->s_Pold.p, by, hipMemcpyDeviceToHost, ctx->st));
+    if (G) HIP_TRY(hipMemcpyAsync(G, ctx->s_G.p, by, hipMemcpyDeviceToHost, ctx->st));
+    HIP_TRY(hipStreamSynchronize(ctx->st));
     return MMD_OK;
 }
 

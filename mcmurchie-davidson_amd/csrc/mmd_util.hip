@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include "mmd_device.h"
 #include "mmd_kernels.h"
+#include "mmd_scf.h"
 
 namespace mmd {
 
@@ -871,6 +872,146 @@ hipError_t launch_jk_tensor(int N, const double* T, const double* Pre, const dou
     if (N <= 0) return hipSuccess;
     hipLaunchKernelGGL(jk_tensor_kernel, dim3((N * N + 255) / 256), dim3(256), 0, st, N, T, Pre, Pim, Jre, Jim, Kre,
                        Kim);
+    return hipGetLastError();
+}
+
+}  // namespace mmd
+
+// ---------------------------------------------------------------------------
+// Elementwise and reduction kernels of the GPU-resident SCF loop
+// (mmd/scf.py:10-180 in the reference; the loop is driven from the host).
+// ---------------------------------------------------------------------------
+namespace mmd {
+
+__global__ void scf_axpby_kernel(size_t n, double a, const double* __restrict__ x, double b,
+                                 const double* __restrict__ y, double* __restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a * x[i] + b * y[i];
+}
+
+hipError_t launch_scf_axpby(size_t n, double a, const double* x, double b, const double* y, double* out,
+                            hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(scf_axpby_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, a, x, b, y, out);
+    return hipGetLastError();
+}
+
+// out = M - M^T (N x N)
+__global__ void scf_antisym_kernel(int N, const double* __restrict__ M, double* __restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)N * N) return;
+    const int r = (int)(i / N), c = (int)(i - (size_t)r * N);
+    out[i] = M[i] - M[(size_t)c * N + r];
+}
+
+hipError_t launch_scf_antisym(int N, const double* M, double* out, hipStream_t st)
+{
+    const size_t n = (size_t)N * N;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(scf_antisym_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, N, M, out);
+    return hipGetLastError();
+}
+
+// One workgroup: out[0] = sum_pq (H + F)_pq P_qp, out[1] = sum (P - P_old)^2
+// (deterministic order).
+__global__ void __launch_bounds__(1024) scf_energy_kernel(int N, const double* __restrict__ H,
+                                                          const double* __restrict__ F,
+                                                          const double* __restrict__ P,
+                                                          const double* __restrict__ Pold, double* out)
+{
+    __shared__ double re[1024], rr[1024];
+    double e = 0.0, r = 0.0;
+    const size_t n = (size_t)N * N;
+    for (size_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const int p = (int)(i / N), q = (int)(i - (size_t)p * N);
+        e = fma(H[i] + F[i], P[(size_t)q * N + p], e);
+        const double d = P[i] - Pold[i];
+        r = fma(d, d, r);
+    }
+    re[threadIdx.x] = e;
+    rr[threadIdx.x] = r;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            re[threadIdx.x] += re[threadIdx.x + s];
+            rr[threadIdx.x] += rr[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = re[0];
+        out[1] = rr[0];
+    }
+}
+
+hipError_t launch_scf_energy(int N, const double* H, const double* F, const double* P, const double* Pold,
+                             double* out, hipStream_t st)
+{
+    hipLaunchKernelGGL(scf_energy_kernel, dim3(1), dim3(1024), 0, st, N, H, F, P, Pold, out);
+    return hipGetLastError();
+}
+
+struct ScfVecs {
+    const double* v[8];
+};
+
+// out[k] = sum_i E[k][i] e[i], k < nset (one workgroup per k; deterministic)
+__global__ void __launch_bounds__(1024) scf_dots_kernel(size_t n, ScfVecs E, const double* __restrict__ e,
+                                                        double* out)
+{
+    __shared__ double red[1024];
+    const double* a = E.v[blockIdx.x];
+    double s = 0.0;
+    for (size_t i = threadIdx.x; i < n; i += blockDim.x) s = fma(a[i], e[i], s);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+hipError_t launch_scf_dots(int nset, size_t n, const double* const* E, const double* e, double* out, hipStream_t st)
+{
+    if (nset <= 0) return hipSuccess;
+    if (nset > 8) return hipErrorInvalidValue;
+    ScfVecs V;
+    for (int k = 0; k < nset; ++k) V.v[k] = E[k];
+    hipLaunchKernelGGL(scf_dots_kernel, dim3(nset), dim3(1024), 0, st, n, V, e, out);
+    return hipGetLastError();
+}
+
+// out = sum_k w[k] F[k], k < nset <= 8
+struct ScfWeights {
+    const double* F[8];
+    double w[8];
+    int n;
+};
+
+__global__ void scf_combine_kernel(size_t n, ScfWeights W, double* __restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int k = 0; k < W.n; ++k) s = fma(W.w[k], W.F[k][i], s);
+    out[i] = s;
+}
+
+hipError_t launch_scf_combine(size_t n, int nset, const double* const* F, const double* w, double* out,
+                              hipStream_t st)
+{
+    if (nset < 1 || nset > 8) return hipErrorInvalidValue;
+    ScfWeights W;
+    for (int k = 0; k < nset; ++k) {
+        W.F[k] = F[k];
+        W.w[k] = w[k];
+    }
+    W.n = nset;
+    hipLaunchKernelGGL(scf_combine_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, W, out);
     return hipGetLastError();
 }
 

@@ -76,6 +76,14 @@ lib.mmd_set_streams.restype = _c_int
 lib.mmd_set_streams.argtypes = [_ctx_p, _c_int]
 lib.mmd_fock_tiers.restype = _c_int
 lib.mmd_fock_tiers.argtypes = [_ctx_p, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64)]
+lib.mmd_scf_begin.restype = _c_int
+lib.mmd_scf_begin.argtypes = [_ctx_p, _c_int, _vp, _vp, _vp, _vp, _vp]
+lib.mmd_scf_step.restype = _c_int
+lib.mmd_scf_step.argtypes = [_ctx_p, _c_double, _c_int, _c_int, _vp]
+lib.mmd_scf_density.restype = _c_int
+lib.mmd_scf_density.argtypes = [_ctx_p, _vp, _vp]
+lib.mmd_scf_get.restype = _c_int
+lib.mmd_scf_get.argtypes = [_ctx_p, _vp, _vp, _vp, _vp]
 lib.mmd_mlp_create.restype = _c_int
 lib.mmd_mlp_create.argtypes = [ctypes.POINTER(_vp), _c_int, _vp, _c_int]
 lib.mmd_mlp_destroy.restype = None
@@ -98,7 +106,7 @@ EXPORTED = [
     "mmd_set_basis", "mmd_set_atoms", "mmd_one_electron", "mmd_schwarz", "mmd_set_schwarz",
     "mmd_eri_tensor", "mmd_eri_list", "mmd_fock_delta", "mmd_jk_tensor", "mmd_boys",
     "mmd_last_fock_ms", "mmd_set_profiling", "mmd_fock_stats", "mmd_set_precision", "mmd_fock_tiers",
-    "mmd_set_streams",
+    "mmd_set_streams", "mmd_scf_begin", "mmd_scf_step", "mmd_scf_density", "mmd_scf_get",
     "mmd_mlp_create", "mmd_mlp_destroy", "mmd_mlp_nparam", "mmd_mlp_set_params", "mmd_mlp_get_params",
     "mmd_mlp_forward", "mmd_mlp_loss_grad", "mmd_mlp_train",
 ]
@@ -286,6 +294,28 @@ class Context:
     def set_streams(self, n):
         """Streams (1..4) the Fock-build classes are spread over."""
         check(lib.mmd_set_streams(self._h, int(n)))
+
+    # -- GPU-resident SCF loop -------------------------------------------------
+    def scf_begin(self, nocc, H, S, X, F, P):
+        """Upload H, S, X, F and P (= P_old) for mmd_scf_step / mmd_scf_density."""
+        arrs = [_f64(np.real(a)) for a in (H, S, X, F, P)]   # kept alive over the call
+        check(lib.mmd_scf_begin(self._h, int(nocc), *[_ptr(a) for a in arrs]))
+
+    def scf_step(self, tol, build, diis, FO):
+        check(lib.mmd_scf_step(self._h, float(tol), 1 if build else 0, 1 if diis else 0, _ptr(FO)))
+        return FO
+
+    def scf_density(self, CO):
+        stats = np.zeros(2)
+        co = _f64(CO)
+        check(lib.mmd_scf_density(self._h, _ptr(co), _ptr(stats)))
+        return float(stats[0]), float(stats[1])
+
+    def scf_get(self):
+        N = self.nbf
+        F, P, Po, G = (np.empty((N, N)) for _ in range(4))
+        check(lib.mmd_scf_get(self._h, _ptr(F), _ptr(P), _ptr(Po), _ptr(G)))
+        return F, P, Po, G
 
     def fock_tiers(self):
         """(surviving shell quartets, of which FP32) of the last symmetrised build."""

@@ -70,7 +70,19 @@ class SCF(object):
         finally:
             self._eigh_threads = None
 
+    # Direct SCFs with at least this many basis functions keep H, S, X, F, P
+    # and the DIIS window in device memory between Fock builds
+    # (mmd_scf_* of libmmd); None runs every step on the host.
+    device_loop_min_nbasis = 100
+
     def _rhf(self, doPrint, DIIS, direct, conver, acc2e, guess='core'):
+        self._rhf_setup(DIIS, direct, acc2e, guess)
+        if (direct and self.device_loop_min_nbasis is not None and self.nbasis >= self.device_loop_min_nbasis
+                and not np.any(np.imag(self.F)) and not np.any(np.imag(self.P))):
+            return self._rhf_loop_device(doPrint, DIIS, conver)
+        return self._rhf_loop(doPrint, DIIS, conver)
+
+    def _rhf_setup(self, DIIS, direct, acc2e, guess):
         self.is_converged = False
         self.delta_energy = 1e20
         self.P_RMS = 1e20
@@ -100,6 +112,7 @@ class SCF(object):
             self.fockSet = []
             self.errorSet = []
 
+    def _rhf_loop(self, doPrint, DIIS, conver):
         for step in range(self.maxiter):
             if step > 0:
                 self.F_old = self.F
@@ -133,16 +146,67 @@ class SCF(object):
                     error = FPS - self.adj(FPS)
                     self.computeDipole()
                     if doPrint:
-                        print("E(SCF)    = ", "{0:.12f}".format(self.energy.real) + " in " + str(step)
-                              + " iterations")
-                        print("  Convergence:")
-                        print("    FPS-SPF  = ", np.linalg.norm(error))
-                        print("    RMS(P)   = ", "{0:.2e}".format(self.P_RMS.real))
-                        print("    dE(SCF)  = ", "{0:.2e}".format(self.delta_energy.real))
-                        print("  Dipole X = ", "{0:.8f}".format(self.mu[0].real))
-                        print("  Dipole Y = ", "{0:.8f}".format(self.mu[1].real))
-                        print("  Dipole Z = ", "{0:.8f}".format(self.mu[2].real))
+                        self._print_converged(step, error)
                     break
+        self.iterations = step
+
+    def _print_converged(self, step, error):
+        print("E(SCF)    = ", "{0:.12f}".format(self.energy.real) + " in " + str(step)
+              + " iterations")
+        print("  Convergence:")
+        print("    FPS-SPF  = ", np.linalg.norm(error))
+        print("    RMS(P)   = ", "{0:.2e}".format(self.P_RMS.real))
+        print("    dE(SCF)  = ", "{0:.2e}".format(self.delta_energy.real))
+        print("  Dipole X = ", "{0:.8f}".format(self.mu[0].real))
+        print("  Dipole Y = ", "{0:.8f}".format(self.mu[1].real))
+        print("  Dipole Z = ", "{0:.8f}".format(self.mu[2].real))
+
+    def _rhf_loop_device(self, doPrint, DIIS, conver):
+        """The loop of _rhf_loop with the N x N matrices in device memory
+        (mmd_scf_*): Fock build, F update, DIIS and the X^T F X products on
+        the GPU; the orthogonalised Fock matrix comes to the host for the same
+        LAPACK eigensolver and its eigenvectors go back.  Same iteration
+        semantics as the host loop (mmd/scf.py:10-80).  After the loop, F, P,
+        P_old, G, C, CO, FO and MO are set as the host loop sets them; the DIIS
+        window stays on the device (fockSet / errorSet are left empty)."""
+        ctx = self.native
+        N = self.nbasis
+        from mmd.integrals.fock import screen_to_matrix
+        if getattr(ctx, "_screen_key", None) != (id(self.screen), len(self.screen)):
+            ctx.set_schwarz(screen_to_matrix(self.screen, N))
+            ctx._screen_key = (id(self.screen), len(self.screen))
+        ctx.scf_begin(self.nocc, self.Core, self.S, self.X, self.F, self.P_old)
+        FO = np.empty((N, N))
+        for step in range(self.maxiter):
+            if step > 0:
+                energy_old = self.energy
+                ctx.scf_step(self.scrTol, True, DIIS, FO)
+                self.fock_ms.append(ctx.last_fock_ms())
+            else:
+                ctx.scf_step(self.scrTol, False, False, FO)
+            with _blas_threads(getattr(self, "_eigh_threads", None)):
+                E, CO = scipy.linalg.eigh(FO, driver="evd")
+            e_el, rms = ctx.scf_density(CO)
+            self.el_energy = np.complex128(e_el)
+            self.energy = self.el_energy + self.nuc_energy
+            if step > 0:
+                self.delta_energy = self.energy - energy_old
+                self.P_RMS = np.float64(rms)
+            if np.abs(self.P_RMS) < conver or step == (self.maxiter - 1):
+                break
+        F, P, P_old, G = ctx.scf_get()
+        self.F, self.P, self.P_old, self.G = (a.astype('complex') for a in (F, P, P_old, G))
+        self.FO, self.CO, self.MO = FO.copy(), CO, E
+        self.C = np.dot(self.X, CO)
+        if step == (self.maxiter - 1):
+            print("NOT CONVERGED")
+        else:
+            self.is_converged = True
+            FPS = np.dot(self.F, np.dot(self.P, self.S))
+            error = FPS - self.adj(FPS)
+            self.computeDipole()
+            if doPrint:
+                self._print_converged(step, error)
         self.iterations = step
 
     def _direct_G(self, P, P_old):

@@ -421,3 +421,27 @@ def test_learned_guess_gain_on_heldout(native):
             it[guess] += m.iterations
     print("iterations over %d held-out clusters: %s" % (len(geoms), it))
     assert it["learned"] < it["sad"]
+
+
+# --------------------------------------------------------------------------
+# The GPU-resident SCF loop (bases of >= 100 functions, mmd_scf_*) steps as
+# the host loop: same iterations, energies within 1e-10 Eh, same density.
+# --------------------------------------------------------------------------
+def test_device_scf_loop_matches_host_loop(native):
+    from mmd.molecule import Molecule
+    geom = water_cluster(5, 6.5, 3)
+    dev = Molecule(geom, "cc-pvdz")
+    assert dev.nbasis >= dev.device_loop_min_nbasis
+    dev.RHF(doPrint=False, direct=True, conver=1e-9)
+    host = Molecule(geom, "cc-pvdz")
+    host.device_loop_min_nbasis = None
+    host.RHF(doPrint=False, direct=True, conver=1e-9)
+    print("device %d / host %d iterations, dE %.2e" % (dev.iterations, host.iterations,
+                                                       abs(dev.energy.real - host.energy.real)))
+    assert dev.is_converged and host.is_converged and dev.iterations == host.iterations
+    assert abs(dev.energy.real - host.energy.real) < 1e-10
+    assert np.max(np.abs(dev.P - host.P)) < 1e-8 and dev.P.dtype == np.complex128
+    assert np.max(np.abs(dev.F - host.F)) < 1e-8 and np.max(np.abs(dev.mu - host.mu)) < 1e-7
+    sad = Molecule(geom, "cc-pvdz")
+    sad.RHF(doPrint=False, direct=True, conver=1e-9, guess="sad")
+    assert abs(sad.energy.real - host.energy.real) < 1e-8

@@ -42,6 +42,10 @@ def _run_one(geom, basis, kwargs, local, device, keep_density):
     mol = Molecule(geom, basis)
     ctx = mol.attach_context(_worker_ctx(local, device))
     ctx.set_precision(kwargs.get("fp32_below", 0.0))
+    if not kwargs.get("device_loop", False):
+        # small molecules side by side: the host loop (the GPU-resident one
+        # measured 179 against 247 SCF/s on 512 (H2O)2-6 / 6-31G)
+        mol.device_loop_min_nbasis = None
     t0 = time.perf_counter()
     mol._rhf(False, kwargs.get("DIIS", True), kwargs.get("direct", True), kwargs.get("conver", 1e-8),
              kwargs.get("acc2e", 1e-12), kwargs.get("guess", "core"))
@@ -56,7 +60,9 @@ def _run_one(geom, basis, kwargs, local, device, keep_density):
 def run_batch(geometries, basis="6-31g", workers=8, device=0, keep_density=False, executor=None, **rhf_kwargs):
     """RHF energies of all geometries.  rhf_kwargs: DIIS, direct (default
     True), conver, acc2e, guess, fp32_below (FP32 tier of the Fock build,
-    0 = all FP64).  Returns one dict per geometry, in order:
+    0 = all FP64), device_loop (default False: the host SCF loop; True: the
+    GPU-resident one).
+    Returns one dict per geometry, in order:
     energy, iterations, converged, nbasis, seconds (and the converged density
     P with keep_density).  ``executor``: a thread pool to reuse (its
     threads keep their GPU contexts between calls)."""

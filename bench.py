@@ -253,6 +253,34 @@ def bench_batch(args):
     }
 
 
+def bench_ao2mo(args):
+    """Four-index transformation of the stored ERI tensor of (H2O)4 / cc-pVDZ
+    (N = 100, 7 A cube, seed 0) on the GPU: four FP64 GEMM quarter
+    transforms, 8 N^5 FLOPs; FP64 peak 78.6 TFLOP/s."""
+    import numpy as np
+    from mmd.molecule import Molecule
+    from mmd.workloads import water_cluster
+    mol = Molecule(water_cluster(4, 7.0, seed=0), "cc-pvdz")
+    mol.RHF(doPrint=False, direct=False)
+    N = mol.nbasis
+    ctx = mol.native
+    C = mol.C.real
+    for _ in range(max(1, args.warmup)):
+        ctx.ao2mo(mol.TwoE, C)
+    ms = []
+    for _ in range(args.steps):
+        ctx.ao2mo(mol.TwoE, C)
+        ms.append(ctx.last_ao2mo_ms)
+    t = float(np.median(ms))
+    flops = 8.0 * N ** 5
+    return {"metric": "four-index transformation (ao2mo), device time of the four GEMM passes", "value": flops / (t * 1e-3) / 1e12,
+            "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "(H2O)4 cc-pVDZ ao2mo", "n_basis": N, "seed": 0},
+            "roofline": {"bound": "mfma", "achieved": flops / (t * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": flops / (t * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, "traffic": None}}
+
+
 def copy_calibration():
     """A device-to-device copy of 256 MiB (five launches), the reference
     point for scaling the HBM byte counters of the profiles."""
@@ -278,7 +306,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
-    ap.add_argument("--workload", choices=("h2o8", "h2o16-tier", "batch512", "copy-calibration"), default="h2o8",
+    ap.add_argument("--workload", choices=("h2o8", "h2o16-tier", "batch512", "ao2mo", "copy-calibration"), default="h2o8",
                     help="h2o8: (H2O)8 cc-pVDZ FP64 (default); h2o16-tier: (H2O)16 cc-pVDZ FP64 vs FP32 tier; "
                          "batch512: 512 small 6-31G clusters")
     args = ap.parse_args()
@@ -301,6 +329,9 @@ def main():
         sys.exit(1)
     if args.workload == "h2o16-tier":
         print(json.dumps(bench_tier(args)))
+        return
+    if args.workload == "ao2mo":
+        print(json.dumps(bench_ao2mo(args)))
         return
 
     geom = water_cluster(8, 7.0, seed=0)

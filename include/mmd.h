@@ -197,6 +197,17 @@ int mmd_scf_density(mmd_ctx *ctx, const double *CO, double *stats);
 int mmd_scf_get(mmd_ctx *ctx, double *F, double *P, double *P_old, double *G);
 
 /*
+ * Four-index transformation of the stored ERI tensor to the molecular-orbital
+ * basis (PostSCF.ao2mo, mmd/postscf.py:21-28):
+ *   out[P,Q,R,S] = sum_pqrs C[p,P] C[q,Q] C[r,R] C[s,S] T[p,q,r,s]
+ * T, out: N^4 row-major (host or device per the flags); C: N*N host.  Four
+ * quarter transforms, each one FP64 GEMM (rocBLAS) of N x N^3 by N x N;
+ * gemm_ms (may be NULL) receives their device time.
+ */
+int mmd_ao2mo(mmd_ctx *ctx, const double *T, int t_on_device, const double *C, double *out, int out_on_device,
+              double *gemm_ms);
+
+/*
  * Multilayer perceptron of the learned initial density (no counterpart in
  * the reference): tanh hidden layers, linear output layer, FP32 HIP kernels.
  *   dims            : nlayer + 1 widths (input, hidden..., output)

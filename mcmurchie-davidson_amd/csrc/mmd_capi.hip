@@ -1176,6 +1176,51 @@ int mmd_scf_get(mmd_ctx* ctx, double* F, double* P, double* P_old, double* G)
     return MMD_OK;
 }
 
+int mmd_ao2mo(mmd_ctx* ctx, const double* T, int t_on_device, const double* C, double* out, int out_on_device,
+              double* gemm_ms)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (ctx->nbf == 0) return fail(MMD_ERR_STATE, "no basis set");
+    if (!T || !C || !out) return fail(MMD_ERR_ARG, "null pointer");
+    const int N = ctx->nbf;
+    const size_t nn = (size_t)N * N, n3 = nn * N, n4 = nn * nn;
+    if (!ctx->blas) {
+        if (rocblas_create_handle(&ctx->blas) != rocblas_status_success)
+            return fail(MMD_ERR_DEVICE, "rocblas_create_handle failed");
+    }
+    if (rocblas_set_stream(ctx->blas, ctx->st) != rocblas_status_success)
+        return fail(MMD_ERR_DEVICE, "rocblas_set_stream failed");
+    if (n3 > (size_t)INT32_MAX) return fail(MMD_ERR_UNSUPPORTED, "basis too large for the transform");
+    DevBuf<double> a, b, dC;
+    HIP_TRY(a.reserve(n4));
+    HIP_TRY(b.reserve(n4));
+    HIP_TRY(dC.reserve(nn));
+    HIP_TRY(hipMemcpyAsync(dC.p, C, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->st));
+    HIP_TRY(hipMemcpyAsync(a.p, T, sizeof(double) * n4, t_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                           ctx->st));
+    // Four quarter transforms.  Each contracts the last index of X (M x N,
+    // M = N^3) with C and writes it first: Y[S][pqr] = sum_s C[s][S] X[pqr][s],
+    // i.e. Y = C^T X^T; after four passes the index order is (P,Q,R,S) again.
+    double* x = a.p;
+    double* y = b.p;
+    HIP_TRY(hipEventRecord(ctx->ev0, ctx->st));
+    for (int pass = 0; pass < 4; ++pass) {
+        if ((rc = gemm_rm(ctx, true, true, N, (int)n3, N, dC.p, N, x, N, y, (int)n3))) return rc;
+        std::swap(x, y);
+    }
+    HIP_TRY(hipEventRecord(ctx->ev1, ctx->st));
+    HIP_TRY(hipMemcpyAsync(out, x, sizeof(double) * n4, out_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                           ctx->st));
+    HIP_TRY(hipStreamSynchronize(ctx->st));
+    if (gemm_ms) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        *gemm_ms = ms;
+    }
+    return MMD_OK;
+}
+
 int mmd_last_fock_ms(mmd_ctx* ctx, double* ms)
 {
     if (!ctx || !ms) return fail(MMD_ERR_ARG, "null pointer");

@@ -84,6 +84,8 @@ lib.mmd_scf_density.restype = _c_int
 lib.mmd_scf_density.argtypes = [_ctx_p, _vp, _vp]
 lib.mmd_scf_get.restype = _c_int
 lib.mmd_scf_get.argtypes = [_ctx_p, _vp, _vp, _vp, _vp]
+lib.mmd_ao2mo.restype = _c_int
+lib.mmd_ao2mo.argtypes = [_ctx_p, _vp, _c_int, _vp, _vp, _c_int, ctypes.POINTER(_c_double)]
 lib.mmd_mlp_create.restype = _c_int
 lib.mmd_mlp_create.argtypes = [ctypes.POINTER(_vp), _c_int, _vp, _c_int]
 lib.mmd_mlp_destroy.restype = None
@@ -106,7 +108,7 @@ EXPORTED = [
     "mmd_set_basis", "mmd_set_atoms", "mmd_one_electron", "mmd_schwarz", "mmd_set_schwarz",
     "mmd_eri_tensor", "mmd_eri_list", "mmd_fock_delta", "mmd_jk_tensor", "mmd_boys",
     "mmd_last_fock_ms", "mmd_set_profiling", "mmd_fock_stats", "mmd_set_precision", "mmd_fock_tiers",
-    "mmd_set_streams", "mmd_scf_begin", "mmd_scf_step", "mmd_scf_density", "mmd_scf_get",
+    "mmd_set_streams", "mmd_scf_begin", "mmd_scf_step", "mmd_scf_density", "mmd_scf_get", "mmd_ao2mo",
     "mmd_mlp_create", "mmd_mlp_destroy", "mmd_mlp_nparam", "mmd_mlp_set_params", "mmd_mlp_get_params",
     "mmd_mlp_forward", "mmd_mlp_loss_grad", "mmd_mlp_train",
 ]
@@ -316,6 +318,20 @@ class Context:
         F, P, Po, G = (np.empty((N, N)) for _ in range(4))
         check(lib.mmd_scf_get(self._h, _ptr(F), _ptr(P), _ptr(Po), _ptr(G)))
         return F, P, Po, G
+
+    def ao2mo(self, T, C, out=None):
+        """(PQ|RS) = sum C_pP C_qQ C_rR C_sS (pq|rs) of an N^4 tensor (GPU GEMMs)."""
+        N = self.nbf
+        t = _f64(T)
+        c = _f64(np.real(C))
+        if t.shape != (N, N, N, N) or c.shape != (N, N):
+            raise ValueError("T must be (N,N,N,N) and C (N,N) with N = %d" % N)
+        if out is None:
+            out = np.empty((N, N, N, N))
+        ms = _c_double(0.0)
+        check(lib.mmd_ao2mo(self._h, _ptr(t), 0, _ptr(c), _ptr(out), 0, ctypes.byref(ms)))
+        self.last_ao2mo_ms = ms.value
+        return out
 
     def fock_tiers(self):
         """(surviving shell quartets, of which FP32) of the last symmetrised build."""

@@ -445,3 +445,24 @@ def test_device_scf_loop_matches_host_loop(native):
     sad = Molecule(geom, "cc-pvdz")
     sad.RHF(doPrint=False, direct=True, conver=1e-9, guess="sad")
     assert abs(sad.energy.real - host.energy.real) < 1e-8
+
+
+# --------------------------------------------------------------------------
+# Four-index transformation (PostSCF.ao2mo, mmd/postscf.py:21-43) on the GPU:
+# against numpy's einsum transform (1e-12) and the reference's MP2 value of
+# tests/test014.py (methane / 3-21G).
+# --------------------------------------------------------------------------
+def test_ao2mo_and_mp2_reference(native):
+    from mmd.molecule import Molecule
+    from mmd.postscf import PostSCF
+    mol = Molecule(METHANE, "3-21g")
+    mol.RHF(doPrint=False, direct=False)
+    PostSCF(mol).MP2()
+    assert abs(mol.energy.real - (-39.9768654272)) < 4e-6
+    assert abs(mol.emp2.real - (-40.076963354817)) < 4e-6
+    C = mol.C.real
+    ref = np.einsum("pqrs,pP,qQ,rR,sS->PQRS", mol.TwoE, C, C, C, C, optimize=True)
+    assert np.max(np.abs(mol.single_bar - ref)) < 1e-12
+    e = mol.emp2.real
+    PostSCF(mol).MP2(spin_orbital=True)
+    assert abs(mol.emp2.real - e) < 1e-10

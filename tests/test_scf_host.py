@@ -64,6 +64,12 @@ class OracleBackedContext:
     def last_fock_ms(self):
         return 0.0
 
+    def ao2mo(self, T, C, out=None):
+        R = np.einsum("pqrs,sS->pqrS", T, C)
+        R = np.einsum("pqrS,rR->pqRS", R, C)
+        R = np.einsum("pqRS,qQ->pQRS", R, C)
+        return np.einsum("pQRS,pP->PQRS", R, C)
+
 
 @pytest.fixture
 def cpu_molecule(monkeypatch, oracle):
@@ -170,3 +176,22 @@ def test_rhf_rejects_unknown_guess(cpu_molecule):
     mol, _ = cpu_molecule(WATER, "sto-3g")
     with pytest.raises(ValueError):
         mol.RHF(doPrint=False, guess="huckel")
+
+
+# -- post-SCF host logic --------------------------------------------------------
+
+def test_mp2_methane_321g_reference_value(cpu_molecule):
+    """tests/test014.py of the reference: E(RHF) and E(MP2) of methane/3-21G
+    (assert_allclose default rtol 1e-7)."""
+    from mmd.postscf import PostSCF
+    mol, _ = cpu_molecule(METHANE, "3-21g")
+    mol.RHF(doPrint=False, direct=False)
+    PostSCF(mol).MP2()
+    assert abs(mol.energy.real - (-39.9768654272)) < 1e-7 * 40
+    assert abs(mol.emp2.real - (-40.076963354817)) < 1e-7 * 40
+    e_spatial = mol.emp2.real
+    PostSCF(mol).MP2(spin_orbital=True)
+    assert abs(mol.emp2.real - e_spatial) < 1e-10
+    n = mol.nbasis
+    assert mol.double_bar.shape == (2 * n,) * 4 and mol.Hp.shape == (2 * n, 2 * n)
+    assert np.allclose(mol.double_bar, -mol.double_bar.transpose(0, 1, 3, 2))

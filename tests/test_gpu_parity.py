@@ -466,3 +466,34 @@ def test_ao2mo_and_mp2_reference(native):
     e = mol.emp2.real
     PostSCF(mol).MP2(spin_orbital=True)
     assert abs(mol.emp2.real - e) < 1e-10
+
+
+# --------------------------------------------------------------------------
+# Real-time TDHF (mmd/realtime.py) on the GPU J/K of complex densities:
+# tests/test011.py of the reference (energy conserved without a field), and
+# the GPU propagation of water under a pulse against the oracle-free host
+# contraction of the same stored tensor.
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("method", ["Magnus2", "Magnus4"])
+def test_realtime_reference_trivial(native, method):
+    from mmd.molecule import Molecule
+    from mmd.realtime import RealTime
+    mol = Molecule("\n0 1\nH 0.0 0.0 0.74\nH 0.0 0.0 0.00\n", "sto-3g")
+    rt = RealTime(mol, numsteps=100, stepsize=0.2, field=0.0001, pulse=None)
+    e0 = rt.mol.energy.real
+    getattr(rt, method)(direction='z')
+    np.testing.assert_allclose(e0 * np.ones_like(rt.Energy), rt.Energy)
+
+
+def test_realtime_complex_jk_vs_einsum(native):
+    from mmd.molecule import Molecule
+    mol = Molecule(WATER, "3-21g")
+    mol.RHF(doPrint=False, direct=False)
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((mol.nbasis, mol.nbasis)) + 1j * rng.standard_normal((mol.nbasis, mol.nbasis))
+    P = 0.5 * (A + A.conj().T) * 0.1
+    mol.P = P
+    mol.buildFock()
+    J = np.einsum("pqrs,sr->pq", mol.TwoE, P)
+    K = np.einsum("psqr,sr->pq", mol.TwoE, P)
+    assert np.max(np.abs(mol.G - (2 * J - K))) < 1e-11

@@ -195,3 +195,40 @@ def test_mp2_methane_321g_reference_value(cpu_molecule):
     n = mol.nbasis
     assert mol.double_bar.shape == (2 * n,) * 4 and mol.Hp.shape == (2 * n, 2 * n)
     assert np.allclose(mol.double_bar, -mol.double_bar.transpose(0, 1, 3, 2))
+
+
+# -- real-time propagation host logic ---------------------------------------------
+
+H2 = """
+0 1
+H 0.0 0.0 0.74
+H 0.0 0.0 0.00
+"""
+
+
+@pytest.mark.parametrize("method", ["Magnus2", "Magnus4"])
+def test_realtime_energy_conserved_without_field(cpu_molecule, method):
+    """tests/test011.py of the reference: with no pulse the energy stays at
+    its SCF value (assert_allclose default rtol 1e-7)."""
+    from mmd.realtime import RealTime
+    mol, _ = cpu_molecule(H2, "sto-3g")
+    rt = RealTime(mol, numsteps=30, stepsize=0.2, field=0.0001, pulse=None)
+    e0 = rt.mol.energy.real
+    getattr(rt, method)(direction='z')
+    np.testing.assert_allclose(np.array(rt.Energy), e0 * np.ones(len(rt.Energy)), rtol=1e-7)
+    assert len(rt.Energy) == 30 and len(rt.dipole) == 30
+
+
+def test_realtime_magnus2_and_magnus4_agree_under_a_pulse(cpu_molecule):
+    """With a weak Gaussian pulse along z both integrators follow the same
+    induced dipole (they differ at O(dt^3) per step); the dipole responds."""
+    from mmd.realtime import RealTime
+    mol, _ = cpu_molecule(WATER, "sto-3g")
+    pulse = lambda t: np.exp(-((t - 0.5) ** 2) / 0.05)
+    rt = RealTime(mol, numsteps=25, stepsize=0.05, field=0.01, pulse=pulse)
+    rt.Magnus2(direction='y')
+    d2 = np.array(rt.dipole)
+    rt.Magnus4(direction='y')
+    d4 = np.array(rt.dipole)
+    assert np.ptp(d4) > 1e-5
+    assert np.max(np.abs(d2 - d4)) < 1e-2 * np.ptp(d4)

@@ -837,32 +837,46 @@ hipError_t launch_sqrt_clamped(size_t n, const double* x, double* y, hipStream_t
 
 // Conventional J/K from the stored tensor (mmd/scf.py:97-98):
 //   J_pq = sum_rs (pq|rs) P_sr ;  K_pq = sum_rs (ps|qr) P_sr.
-__global__ void jk_tensor_kernel(int N, const double* __restrict__ T, const double* __restrict__ Pre,
-                                 const double* __restrict__ Pim, double* Jre, double* Jim, double* Kre, double* Kim)
+// One wavefront per (p,q): the lanes run along the tensor's contiguous last
+// index, so that both the row (pq|..) of J and the N segments (ps|q.) of K
+// are read coalesced; the tensor is read twice in all (HBM-bound).
+__global__ void __launch_bounds__(256) jk_tensor_kernel(int N, const double* __restrict__ T,
+                                                        const double* __restrict__ Pre,
+                                                        const double* __restrict__ Pim, double* Jre, double* Jim,
+                                                        double* Kre, double* Kim)
 {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= N * N) return;
-    const int p = idx / N, q = idx % N;
-    const size_t n = (size_t)N;
+    const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (w >= (int64_t)N * N) return;   // uniform across the wavefront
+    const int p = (int)(w / N), q = (int)(w - (int64_t)p * N);
+    const size_t n = (size_t)N, nn = n * n;
     double jr = 0, ji = 0, kr = 0, ki = 0;
+    const double* row = T + (size_t)w * nn;                  // (pq|r s) at row[r*N + s]
     for (int r = 0; r < N; ++r)
-        for (int s = 0; s < N; ++s) {
-            const double tj = T[((p * n + q) * n + r) * n + s];
-            const double tk = T[((p * n + s) * n + q) * n + r];
-            const double pr = Pre[s * n + r];
-            jr += tj * pr;
-            kr += tk * pr;
-            if (Pim) {
-                const double pi = Pim[s * n + r];
-                ji += tj * pi;
-                ki += tk * pi;
-            }
+        for (int s = lane; s < N; s += 64) {
+            const double t = row[(size_t)r * n + s];
+            jr = fma(t, Pre[(size_t)s * n + r], jr);
+            if (Pim) ji = fma(t, Pim[(size_t)s * n + r], ji);
         }
-    Jre[idx] = jr;
-    Kre[idx] = kr;
+    const double* kb = T + (size_t)p * n * nn + (size_t)q * n; // (p s|q r) at kb[s*N*N + r]
+    for (int s2 = 0; s2 < N; ++s2)
+        for (int r2 = lane; r2 < N; r2 += 64) {
+            const double t = kb[(size_t)s2 * nn + r2];
+            kr = fma(t, Pre[(size_t)s2 * n + r2], kr);
+            if (Pim) ki = fma(t, Pim[(size_t)s2 * n + r2], ki);
+        }
+    for (int off = 32; off > 0; off >>= 1) {
+        jr += __shfl_xor(jr, off);
+        kr += __shfl_xor(kr, off);
+        ji += __shfl_xor(ji, off);
+        ki += __shfl_xor(ki, off);
+    }
+    if (lane != 0) return;
+    Jre[w] = jr;
+    Kre[w] = kr;
     if (Pim) {
-        Jim[idx] = ji;
-        Kim[idx] = ki;
+        Jim[w] = ji;
+        Kim[w] = ki;
     }
 }
 
@@ -870,8 +884,9 @@ hipError_t launch_jk_tensor(int N, const double* T, const double* Pre, const dou
                             double* Kre, double* Kim, hipStream_t st)
 {
     if (N <= 0) return hipSuccess;
-    hipLaunchKernelGGL(jk_tensor_kernel, dim3((N * N + 255) / 256), dim3(256), 0, st, N, T, Pre, Pim, Jre, Jim, Kre,
-                       Kim);
+    const int64_t waves = (int64_t)N * N;   // four wavefronts per 256-thread block
+    hipLaunchKernelGGL(jk_tensor_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, N, T, Pre, Pim, Jre,
+                       Jim, Kre, Kim);
     return hipGetLastError();
 }
 

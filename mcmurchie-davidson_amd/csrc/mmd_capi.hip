@@ -113,6 +113,7 @@ struct mmd_ctx {
     int pc_off[NPAIRCLASS + 1] = {0};
     DevBuf<ShellPair> d_pairs;
     DevBuf<PrimPair> d_pp;
+    int64_t npp_total = 0;
 
     // Schwarz
     bool schwarz_ok = false;
@@ -328,6 +329,7 @@ int build_pairs(mmd_ctx* c, const std::vector<double>* qpair)
         c->pairs.insert(c->pairs.end(), v.begin(), v.end());
     }
     c->pc_off[NPAIRCLASS] = (int)c->pairs.size();
+    c->npp_total = (int64_t)pp.size();
     HIP_TRY(c->d_pairs.reserve(c->pairs.size()));
     HIP_TRY(c->d_pp.reserve(pp.size()));
     HIP_TRY(hipMemcpyAsync(c->d_pairs.p, c->pairs.data(), sizeof(ShellPair) * c->pairs.size(), hipMemcpyHostToDevice,
@@ -767,6 +769,53 @@ int mmd_set_schwarz(mmd_ctx* ctx, const double* diag)
     return set_schwarz(ctx, h);
 }
 
+namespace {
+// The full (ij|kl) tensor from the shell-pair tables: every quartet class
+// with bra pair class >= ket pair class, all canonical shell quartets of it
+// (bra pair >= ket pair inside one pair class), every integral stored at its
+// eight index permutations.  Every element of dst is written.
+hipError_t eri_tensor_shells(mmd_ctx* c, double* dst)
+{
+    ClassArgs A = base_args(c);
+    A.G = dst;
+    size_t bps = 0;
+    for (int qc = 0; qc < NQCLASS; ++qc) bps = std::max(bps, (size_t)tensor_class_bps(qc));
+    DevBuf<double> table;
+    hipError_t e = table.reserve(std::max<size_t>(1, (size_t)c->npp_total * bps));
+    if (e != hipSuccess) return e;
+    for (int qc = 0; qc < NQCLASS; ++qc) {
+        int pcb, pck, la, lb, lc, ld;
+        class_of(qc, &pcb, &pck);
+        qclass_decode(qc, &la, &lb, &lc, &ld);
+        const int boff = c->pc_off[pcb], nb = c->pc_off[pcb + 1] - boff;
+        const int koff = c->pc_off[pck], nk = c->pc_off[pck + 1] - koff;
+        if (nb == 0 || nk == 0) continue;
+        const int tri = pcb == pck;
+        if (fock_class_is_wave(la, lb, lc, ld)) {
+            std::vector<int2> q;
+            q.reserve((size_t)nb * (nb + 1) / 2);
+            for (int P = 0; P < nb; ++P)
+                for (int Q = 0; Q <= P; ++Q) q.push_back(make_int2(boff + P, koff + Q));
+            const int n = (int)q.size();
+            e = c->d_surv.reserve(q.size());
+            if (e == hipSuccess) e = hipMemcpyAsync(c->d_surv.p, q.data(), sizeof(int2) * q.size(),
+                                                    hipMemcpyHostToDevice, c->st);
+            if (e == hipSuccess) e = hipMemcpyAsync(c->d_count.p + qc, &n, sizeof(int), hipMemcpyHostToDevice, c->st);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->st);   // q and n leave scope
+            if (e != hipSuccess) return e;
+            ClassArgs W = A;
+            W.quart = c->d_surv.p;
+            W.count = c->d_count.p + qc;
+            e = launch_tensor_wave_class(W, n, c->st);
+        } else {
+            e = launch_tensor_class(qc, A, boff, nb, koff, nk, tri, table.p, dst, c->st);
+        }
+        if (e != hipSuccess) return e;
+    }
+    return hipStreamSynchronize(c->st);   // before the table is freed
+}
+}  // namespace
+
 int mmd_eri_tensor(mmd_ctx* ctx, double* out, int out_on_device)
 {
     int rc = check_ctx(ctx);
@@ -780,8 +829,16 @@ int mmd_eri_tensor(mmd_ctx* ctx, double* out, int out_on_device)
         HIP_TRY(tmp.reserve(n4));
         dst = tmp.p;
     }
-    HIP_TRY(launch_canonical_quartets(0, ctx->nbf, ctx->d_bf.p, ctx->d_bf_exp.p, ctx->d_bf_w.p, ctx->d_boys.p, nullptr,
-                                      nullptr, nullptr, 0.0, dst, ctx->st));
+    if (ctx->shells_ok) {
+        if (!ctx->pairs_ok) {
+            rc = build_pairs(ctx, nullptr);
+            if (rc) return rc;
+        }
+        HIP_TRY(eri_tensor_shells(ctx, dst));
+    } else {
+        HIP_TRY(launch_canonical_quartets(0, ctx->nbf, ctx->d_bf.p, ctx->d_bf_exp.p, ctx->d_bf_w.p, ctx->d_boys.p,
+                                          nullptr, nullptr, nullptr, 0.0, dst, ctx->st));
+    }
     if (!out_on_device) HIP_TRY(hipMemcpyAsync(out, dst, sizeof(double) * n4, hipMemcpyDeviceToHost, ctx->st));
     HIP_TRY(hipStreamSynchronize(ctx->st));
     return MMD_OK;

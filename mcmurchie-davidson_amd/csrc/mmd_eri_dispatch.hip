@@ -18,6 +18,37 @@ namespace mmd {
 MMD_CLASSES(DECL)
 #undef DECL
 
+#define DECLT(a, b, c, d)                                                                                    \
+    hipError_t tensor_class_launch_##a##b##c##d##_d(const ClassArgs& A, int boff, int nb, int koff, int nk,  \
+                                                    int tri, double* table, double* out, hipStream_t st);    \
+    int tensor_class_bps_##a##b##c##d##_d();
+MMD_CLASSES(DECLT)
+#undef DECLT
+
+hipError_t launch_tensor_class(int qc, const ClassArgs& A, int boff, int nb, int koff, int nk, int tri,
+                               double* table, double* out, hipStream_t st)
+{
+    int la, lb, lc, ld;
+    qclass_decode(qc, &la, &lb, &lc, &ld);
+#define CASE(a, b, c, d)                          \
+    if (la == a && lb == b && lc == c && ld == d) \
+        return tensor_class_launch_##a##b##c##d##_d(A, boff, nb, koff, nk, tri, table, out, st);
+    MMD_CLASSES(CASE)
+#undef CASE
+    return hipErrorInvalidValue;
+}
+
+int tensor_class_bps(int qc)
+{
+    int la, lb, lc, ld;
+    qclass_decode(qc, &la, &lb, &lc, &ld);
+#define CASE(a, b, c, d) \
+    if (la == a && lb == b && lc == c && ld == d) return tensor_class_bps_##a##b##c##d##_d();
+    MMD_CLASSES(CASE)
+#undef CASE
+    return 0;
+}
+
 hipError_t launch_fock_class(int qc, int fp32, const ClassArgs& A, const RowArgs& R, int grid, hipStream_t st)
 {
     int la, lb, lc, ld;

@@ -561,6 +561,103 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
     }
 }
 
+// ---------------------------------------------------------------------------
+// Stored ERI tensor (doERIs, cython/twoe.pyx:9-31) from shell quartets.
+// bra_table_kernel writes the staged bra data of stage_bra for every
+// primitive pair of the class's bra pairs to global memory (table + pp*BPS);
+// eri_tensor_class_kernel gives one lane per (shell quartet, bra-component
+// chunk), contracts all primitive pairs with quartet_compute and stores each
+// canonical integral at its eight index permutations (cython/twoe.pyx:23-30),
+// so that every element of the tensor is written once and the tensor has the
+// permutational symmetry exactly.
+// ---------------------------------------------------------------------------
+template <int LA, int LB, int LC, int LD>
+__global__ void __launch_bounds__(256) bra_table_kernel(ClassArgs A, int boff, double* table)
+{
+    using S = ClassShape<LA, LB, LC, LD>;
+    const ShellPair bra = A.pairs[boff + blockIdx.x];
+    for (int idx = threadIdx.x; idx < 3 * bra.npp; idx += blockDim.x) {
+        const int ip = idx / 3, x = idx - 3 * ip;
+        const PrimPair* P = A.pp + bra.pp0 + ip;
+        double* b = table + (size_t)(bra.pp0 + ip) * S::BPS;
+        if (x == 0) {
+            b[0] = P->p;
+            b[1] = TWO_PI_2_5 * P->K * P->invp;
+            b[2] = P->Px;
+            b[3] = P->Py;
+            b[4] = P->Pz;
+        }
+        double E[S::EB];
+        hermite_E<LA, LB, 1>(P->inv2p, x == 0 ? P->PAx : (x == 1 ? P->PAy : P->PAz),
+                             x == 0 ? P->PBx : (x == 1 ? P->PBy : P->PBz), E);
+        sfor<S::EB>([&](auto k) { b[8 + x * S::EB + k] = E[k]; });
+    }
+}
+
+template <int LA, int LB, int LC, int LD>
+__global__ void __launch_bounds__(256) eri_tensor_class_kernel(ClassArgs A, int boff, int nb, int koff, int nk,
+                                                               int tri, const double* __restrict__ table,
+                                                               double* __restrict__ out)
+{
+    using S = ClassShape<LA, LB, LC, LD>;
+    __shared__ double lds_boys[BOYS_TAYLOR * BOYS_NT];
+    boys_stage_lds<S::L, double>(lds_boys, A.boys);
+    __syncthreads();
+    const int64_t nq = tri ? (int64_t)nb * (nb + 1) / 2 : (int64_t)nb * nk;
+    const int64_t items = nq * S::NCHUNK;
+    const size_t N = (size_t)A.N;
+    for (int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; it < items;
+         it += (int64_t)gridDim.x * blockDim.x) {
+        const int chunk = (int)(it / nq);   // uniform across most wavefronts
+        const int64_t q = it - (int64_t)chunk * nq;
+        int P, Q;
+        if (tri) {
+            int64_t r = (int64_t)((sqrt(8.0 * (double)q + 1.0) - 1.0) * 0.5);
+            while (r * (r + 1) / 2 > q) --r;
+            while ((r + 1) * (r + 2) / 2 <= q) ++r;
+            P = (int)r;
+            Q = (int)(q - r * (r + 1) / 2);
+        } else {
+            P = (int)(q / nk);
+            Q = (int)(q - (int64_t)P * nk);
+        }
+        const ShellPair bra = A.pairs[boff + P];
+        const ShellPair ket = A.pairs[koff + Q];
+        static_switch<0, S::NCHUNK>(chunk, [&](auto ic) {
+            constexpr int C = decltype(ic)::value;
+            constexpr int A0 = C * S::CH, A1 = (C + 1) * S::CH;
+            double acc[(A1 - A0) * S::NCD];
+            quartet_compute<LA, LB, LC, LD, A0, A1, double>(A, lds_boys, table + (size_t)bra.pp0 * S::BPS, ket, 0,
+                                                            bra.npp, 0, ket.npp, acc);
+            sfor<A1 - A0>([&](auto ii) {
+                constexpr int ab = A0 + decltype(ii)::value;
+                constexpr int a = ab / S::NB, b = ab % S::NB;
+                const size_t i = bra.bfa + a, j = bra.bfb + b;
+                const double gab = A.gcomp[i] * A.gcomp[j];
+                sfor<S::NCD>([&](auto icd) {
+                    constexpr int cd = decltype(icd)::value;
+                    constexpr int c = cd / S::ND, d = cd % S::ND;
+                    const size_t k = ket.bfa + c, l = ket.bfb + d;
+                    const size_t ij = i * N + j, ji = j * N + i, kl = k * N + l, lk = l * N + k;
+                    // one producer per orbit of the eight permutations: the
+                    // others (same shell, or the same pair on both sides)
+                    // hold the same integral with different rounding
+                    if ((bra.same && i < j) || (ket.same && k < l) || (tri && P == Q && ij < kl)) return;
+                    const double v = acc[decltype(ii)::value * S::NCD + cd] * gab * A.gcomp[k] * A.gcomp[l];
+                    out[ij * N * N + kl] = v;
+                    out[ji * N * N + kl] = v;
+                    out[ij * N * N + lk] = v;
+                    out[ji * N * N + lk] = v;
+                    out[kl * N * N + ij] = v;
+                    out[lk * N * N + ij] = v;
+                    out[kl * N * N + ji] = v;
+                    out[lk * N * N + ji] = v;
+                });
+            });
+        });
+    }
+}
+
 template <int LA, int LB, int LC, int LD, typename T>
 inline size_t fock_class_lds_bytes(int N, int maxpp)
 {

@@ -27,6 +27,13 @@ hipError_t launch_fock_class(int qc, int fp32, const ClassArgs& A, const RowArgs
 size_t fock_class_lds(int qc, int fp32, int N, int maxpp);
 hipError_t launch_fock_wave_class(int la, int lb, int lc, int ld, const ClassArgs& A, int64_t max_quartets,
                                   hipStream_t st);
+// Stored ERI tensor from shell quartets: class kernels (every class but
+// (dd|dd)) and the wavefront kernel for (dd|dd).  table: staged bra data,
+// tensor_class_bps(qc) doubles per primitive pair.
+hipError_t launch_tensor_class(int qc, const ClassArgs& A, int boff, int nb, int koff, int nk, int tri,
+                               double* table, double* out, hipStream_t st);
+int tensor_class_bps(int qc);
+hipError_t launch_tensor_wave_class(const ClassArgs& A, int64_t nquartets, hipStream_t st);
 // Classes served by the wavefront-per-quartet kernel instead of a unrolled class kernel.
 inline bool fock_class_is_wave(int la, int lb, int lc, int ld) { return la == 2 && lb == 2 && lc == 2 && ld == 2; }
 

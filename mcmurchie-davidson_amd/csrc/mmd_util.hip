@@ -573,8 +573,10 @@ __global__ void __launch_bounds__(64 * GW_WAVES) fock_wave_class_kernel(ClassArg
 // F_m, the R_tuv levels are computed in parallel across the wavefront
 // through two LDS buffers, the E tables by six lanes; the ket contraction
 // W_tuv(cd) is spread over the lanes into LDS, then every lane applies the
-// bra expansion to its component quartets.
-template <int LA, int LB, int LC, int LD>
+// bra expansion to its component quartets.  TENSOR: the integrals are
+// stored at their eight index permutations in A.G (an N^4 tensor) instead
+// of being digested into a Fock matrix.
+template <int LA, int LB, int LC, int LD, bool TENSOR = false>
 __global__ void __launch_bounds__(64 * GW_WAVES) fock_wave_class_kernel_t(ClassArgs A)
 {
     constexpr int L = LA + LB + LC + LD;
@@ -702,6 +704,29 @@ __global__ void __launch_bounds__(64 * GW_WAVES) fock_wave_class_kernel_t(ClassA
             }
         }
         const int bfa = bra.bfa, bfb = bra.bfb, bfc = ket.bfa, bfd = ket.bfb;
+        if constexpr (TENSOR) {
+            const size_t n = (size_t)N;
+#pragma unroll
+            for (int i = 0; i < NCPL; ++i) {
+                const int k = lane + 64 * i;
+                if (k >= NTOT) continue;
+                const int ab = k / NCD, cd = k % NCD;
+                const size_t p = bfa + ab / NB, qq = bfb + ab % NB, r = bfc + cd / ND, s = bfd + cd % ND;
+                const size_t pq = p * n + qq, qp = qq * n + p, rs = r * n + s, sr = s * n + r;
+                // one producer per orbit of the eight permutations
+                if ((bra.same && p < qq) || (ket.same && r < s) || (qd.x == qd.y && pq < rs)) continue;
+                const double v = acc[i] * A.gcomp[p] * A.gcomp[qq] * A.gcomp[r] * A.gcomp[s];
+                A.G[pq * n * n + rs] = v;
+                A.G[qp * n * n + rs] = v;
+                A.G[pq * n * n + sr] = v;
+                A.G[qp * n * n + sr] = v;
+                A.G[rs * n * n + pq] = v;
+                A.G[sr * n * n + pq] = v;
+                A.G[rs * n * n + qp] = v;
+                A.G[sr * n * n + qp] = v;
+            }
+            continue;
+        }
         const double deg = (bra.same ? 1.0 : 2.0) * (ket.same ? 1.0 : 2.0) * (qd.x == qd.y ? 1.0 : 2.0);
 #pragma unroll
         for (int i = 0; i < NCPL; ++i) {
@@ -738,6 +763,18 @@ hipError_t launch_fock_wave_class(int la, int lb, int lc, int ld, const ClassArg
         hipLaunchKernelGGL((fock_wave_class_kernel_t<2, 2, 2, 2>), dim3((unsigned)grid), dim3(64 * GW_WAVES), 0, st, A);
     else
         hipLaunchKernelGGL(fock_wave_class_kernel, dim3((unsigned)grid), dim3(64 * GW_WAVES), 0, st, A, la, lb, lc, ld);
+    return hipGetLastError();
+}
+
+// Stored-tensor mode of the (dd|dd) class: A.quart lists the canonical shell
+// quartets (*A.count of them) and A.G is the N^4 output tensor.
+hipError_t launch_tensor_wave_class(const ClassArgs& A, int64_t nquartets, hipStream_t st)
+{
+    if (nquartets <= 0) return hipSuccess;
+    int64_t grid = (nquartets + GW_WAVES - 1) / GW_WAVES;
+    if (grid > 4096) grid = 4096;
+    hipLaunchKernelGGL((fock_wave_class_kernel_t<2, 2, 2, 2, true>), dim3((unsigned)grid), dim3(64 * GW_WAVES), 0, st,
+                       A);
     return hipGetLastError();
 }
 

@@ -129,6 +129,32 @@ def test_eri_list_vs_oracle_cluster(native, oracle):
     assert np.max(np.abs(got - ref)) < 1e-11
 
 
+def test_eri_tensor_cluster_vs_generic_and_oracle(native, oracle):
+    """(H2O)2/cc-pVDZ stored tensor (shell-quartet class kernels, (dd|dd) on
+    the wavefront kernel): every element against the per-function generic
+    kernel, random and d-only elements against the oracle, 1e-11 absolute."""
+    geom = water_cluster(2, 7.0, 0)
+    from mmd.molecule import Molecule
+    mol = Molecule(geom, "cc-pvdz")
+    mol.two_electron_integrals()
+    T = mol.TwoE
+    N = mol.nbasis
+    for perm in [(1, 0, 2, 3), (0, 1, 3, 2), (2, 3, 0, 1), (3, 2, 1, 0)]:
+        assert np.array_equal(T, T.transpose(perm))
+    for i in range(N):   # every element, one leading index at a time
+        idx = np.indices((1, N, N, N)).reshape(4, -1).T
+        idx[:, 0] = i
+        got = mol.native.eri_list(idx)
+        assert np.max(np.abs(T[i].ravel() - got)) < 1e-11, i
+    om = oracle.OracleMolecule(geom, _gbs("cc-pvdz"))
+    rng = np.random.default_rng(5)
+    dfun = np.array([k for k, b in enumerate(mol.bfs) if sum(b.shell) == 2])
+    assert len(dfun) == 12
+    for idx in (rng.integers(0, N, size=(3000, 4)), dfun[rng.integers(0, len(dfun), size=(3000, 4))]):
+        ref = om.eri_list(idx)
+        assert np.max(np.abs(T[idx[:, 0], idx[:, 1], idx[:, 2], idx[:, 3]] - ref)) < 1e-11
+
+
 def test_schwarz_vs_oracle(native, oracle):
     from mmd.molecule import Molecule
     geom = water_cluster(2, 7.0, 3)

@@ -137,6 +137,13 @@ struct mmd_ctx {
     hipEvent_t evc[2 * NQCLASS] = {};
     DevBuf<unsigned long long> d_nprim;   // [0,21): primitive quartets, [21,42): shell quartets
     DevBuf<int> d_rowctr;
+    // Work units of the row-kernel classes (built once per pair table):
+    // class qc owns units [unit_off[qc], unit_off[qc+1]) with ket segment
+    // length unit_seg[qc].
+    DevBuf<int4> d_units;
+    int64_t unit_off[NQCLASS + 1] = {0};
+    int unit_seg[NQCLASS] = {0};
+    bool units_ok = false;
     size_t max_lds = 65536;
     bool last_row_kernel[NQCLASS] = {};
     int64_t st_nquart[NQCLASS] = {0};
@@ -350,6 +357,61 @@ int build_pairs(mmd_ctx* c, const std::vector<double>* qpair)
     HIP_TRY(c->d_surv.reserve((size_t)std::max<int64_t>(c->qc_off[NQCLASS], (int64_t)c->pairs.size())));
     HIP_TRY(c->d_count.reserve(NQCLASS));
     c->pairs_ok = true;
+    c->units_ok = false;
+    return MMD_OK;
+}
+
+// Work units of every class: one per (bra pair, slice of at most PSLICE of
+// its primitive pairs, segment of seg ket pairs), in decreasing order of
+// estimated cost (bra primitive pairs x ket primitive pairs of the
+// segment), so that the heaviest units start first.  Slicing the bra
+// primitives spreads the heavily contracted pairs over several workgroups.
+void class_of(int qc, int* pcb, int* pck);
+
+int build_units(mmd_ctx* c)
+{
+    std::vector<int4> all;
+    std::vector<std::pair<double, int4>> v;
+    std::vector<double> kpre;
+    for (int qc = 0; qc < NQCLASS; ++qc) {
+        c->unit_off[qc] = (int64_t)all.size();
+        int pcb, pck;
+        class_of(qc, &pcb, &pck);
+        const int boff = c->pc_off[pcb], nb = c->pc_off[pcb + 1] - boff;
+        const int koff = c->pc_off[pck], nk = c->pc_off[pck + 1] - koff;
+        const bool tri = pcb == pck;
+        const int64_t quartets = tri ? (int64_t)nb * (nb + 1) / 2 : (int64_t)nb * nk;
+        // ket segment: long (survivors are batched inside a unit), but short
+        // enough for a few thousand work units per class
+        int seg = 4096;
+        while (seg > 256 && quartets / seg < 4096) seg /= 2;
+        c->unit_seg[qc] = seg;
+        if (nb == 0 || nk == 0) continue;
+        kpre.assign(nk + 1, 0.0);
+        for (int k = 0; k < nk; ++k) kpre[k + 1] = kpre[k] + c->pairs[koff + k].npp;
+        v.clear();
+        for (int P = 0; P < nb; ++P) {
+            const int npp = c->pairs[boff + P].npp;
+            const int nkP = tri ? P + 1 : nk;
+            for (int k0 = 0; k0 < nkP; k0 += seg) {
+                const double kc = kpre[std::min(nkP, k0 + seg)] - kpre[k0];
+                for (int ip = 0; ip < npp; ip += PSLICE) {
+                    const int ip1 = std::min(npp, ip + PSLICE);
+                    v.push_back({(ip1 - ip) * kc, make_int4(P, ip, ip1, k0)});
+                }
+            }
+        }
+        std::stable_sort(v.begin(), v.end(), [](const std::pair<double, int4>& x, const std::pair<double, int4>& y) {
+            return x.first > y.first;
+        });
+        for (auto& e : v) all.push_back(e.second);
+    }
+    c->unit_off[NQCLASS] = (int64_t)all.size();
+    HIP_TRY(c->d_units.reserve(std::max<size_t>(1, all.size())));
+    if (!all.empty())
+        HIP_TRY(hipMemcpyAsync(c->d_units.p, all.data(), sizeof(int4) * all.size(), hipMemcpyHostToDevice, c->st));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    c->units_ok = true;
     return MMD_OK;
 }
 
@@ -393,6 +455,10 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol)
     HIP_TRY(launch_fill_int(NQCLASS, c->d_count.p, 0, c->st));
     HIP_TRY(launch_fill_int(6 * NQCLASS, reinterpret_cast<int*>(c->d_nprim.p), 0, c->st));
     const bool tier32 = c->fp32_below > tol;
+    if (!c->units_ok) {
+        const int rc = build_units(c);
+        if (rc) return rc;
+    }
     bool row_kernel[NQCLASS];
     int maxpp[NQCLASS];
     for (int qc = 0; qc < NQCLASS; ++qc) {
@@ -459,13 +525,14 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol)
             R.Dsh = c->d_Dsh.p;
             R.row_counter = c->d_rowctr.p + qc;
             R.maxpp = maxpp[qc];
-            // ket segment: long (survivors are batched inside a unit), but
-            // short enough for a few thousand work units per class
-            const int64_t quartets = R.tri ? (int64_t)nb * (nb + 1) / 2 : (int64_t)nb * nk;
-            int seg = 4096;
-            while (seg > 256 && quartets / seg < 4096) seg /= 2;
-            R.seg = seg;
-            const int64_t units = (int64_t)nb * ((nk + seg - 1) / seg);
+            R.seg = c->unit_seg[qc];
+            R.units = c->d_units.p + c->unit_off[qc];
+            const int64_t units = c->unit_off[qc + 1] - c->unit_off[qc];
+            R.nunits = (int)units;
+            if (units == 0) {
+                if (c->profiling) HIP_TRY(hipEventRecord(c->evc[2 * qc + 1], sq));
+                continue;
+            }
             R.stat_nquart = c->d_nprim.p + NQCLASS + qc;
             R.stat_nprim = c->profiling ? c->d_nprim.p + qc : nullptr;
             R.tier_lo = tier32 ? c->fp32_below : 0.0;

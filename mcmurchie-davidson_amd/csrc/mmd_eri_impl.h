@@ -74,15 +74,16 @@ __device__ __forceinline__ double dmax6(double a, double b, double c, double d, 
     return fmax(fmax(fmax(a, b), fmax(c, d)), fmax(e, f));
 }
 
-// Stage the bra pair's primitive pairs in LDS: p, 2 pi^{5/2} K / p, P and the
-// three 1-D Hermite tables E^{ab}_t (cython/util.pxi:13-28).
+// Stage primitive pairs ip0 .. ip0+np-1 of the bra pair in LDS: p,
+// 2 pi^{5/2} K / p, P and the three 1-D Hermite tables E^{ab}_t
+// (cython/util.pxi:13-28).
 template <int LA, int LB, int LC, int LD, typename T>
-__device__ __forceinline__ void stage_bra(const ClassArgs& A, const ShellPair& bra, T* bp)
+__device__ __forceinline__ void stage_bra(const ClassArgs& A, const ShellPair& bra, int ip0, int np, T* bp)
 {
     using S = ClassShape<LA, LB, LC, LD>;
-    for (int idx = threadIdx.x; idx < 3 * bra.npp; idx += blockDim.x) {
+    for (int idx = threadIdx.x; idx < 3 * np; idx += blockDim.x) {
         const int ip = idx / 3, x = idx - 3 * ip;
-        const PrimPair* P = A.pp + bra.pp0 + ip;
+        const PrimPair* P = A.pp + bra.pp0 + ip0 + ip;
         T* b = bp + ip * S::BPS;
         if (x == 0) {
             b[0] = (T)P->p;
@@ -367,22 +368,22 @@ __device__ __forceinline__ void static_switch(int i, F&& f)
 }
 
 // ---------------------------------------------------------------------------
-// Class kernel.  A work unit is one bra shell pair ("row") and a segment of
-// its ket pairs; workgroups take units from a device counter.  The unit's
-// ket pairs are screened a workgroup at a time; survivors are appended to an
-// LDS list that is digested whenever it holds a full batch, as wave items of
-// (64 quartets, bra-component chunk, bra primitive slice, ket primitive
-// slice), so that lanes stay busy and heavily contracted quartets are spread
-// over several lanes.  Classes with an integral buffer (IBUF) add each
-// item's partial integrals into LDS and digest every quartet once, after the
-// batch; the others digest every item.  Coulomb G_ab and the four exchange
+// Class kernel.  A work unit is one bra shell pair ("row"), a slice of at
+// most PSLICE of its primitive pairs and a segment of its ket pairs (a
+// host-built table, heaviest first); workgroups take units from a device
+// counter.  The unit's ket pairs are screened a workgroup at a time;
+// survivors are appended to an LDS list that is digested whenever it holds
+// a full batch, as wave items of (bra-component chunk, 64 consecutive
+// (ket primitive slice, quartet) lane items).  Classes with an integral
+// buffer (IBUF) slice the ket primitives too, add each item's partial
+// integrals into LDS and digest every quartet once, after the batch; the
+// others contract a quartet's whole ket side in one lane and digest it.  Coulomb G_ab and the four exchange
 // blocks, all in the rows of the bra shells, are accumulated in LDS
 // (ds_add_f64) and their nonzero entries added to G once per unit; only the
 // ket-side Coulomb block is added to G directly.  (The host sends a class
 // whose bra rows do not fit in LDS to the wavefront-per-quartet kernel.)
 // ---------------------------------------------------------------------------
 constexpr int CLASS_BLOCK = 256;           // threads per workgroup
-constexpr int PSLICE = 8;                  // primitive pairs per slice (bra and ket)
 
 template <int QB>
 struct ClassList {
@@ -401,7 +402,7 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
     double* ibuf = rows + (S::NA + S::NB) * A.N;
     T* lds_boys = reinterpret_cast<T*>(ibuf + S::IBUF_DOUBLES);
     T* bp = lds_boys + BOYS_TAYLOR * BOYS_NT;
-    int* qlist = reinterpret_cast<int*>(bp + (size_t)R.maxpp * S::BPS);
+    int* qlist = reinterpret_cast<int*>(bp + (size_t)(R.maxpp < PSLICE ? R.maxpp : PSLICE) * S::BPS);
     int* misc = qlist + ClassList<QB>::QLIST;
     if constexpr (std::is_same<T, float>::value)
         boys_stage_lds<S::L, float>(lds_boys, A.boysf);
@@ -412,8 +413,7 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
     const int N = A.N;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int nwv = (int)(blockDim.x >> 6);
-    const int upr = (R.nk + R.seg - 1) / R.seg;          // units per row
-    const int nunits = R.nb * upr;
+    const int nunits = R.nunits;
     const size_t ns = (size_t)R.nshell;
     unsigned long long my_nprim = 0;   // primitive quartets of this lane (statistics)
     for (;;) {
@@ -428,24 +428,24 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
             }
             break;
         }
-        // triangular classes hand out their long rows first
-        const int P = R.tri ? (R.nb - 1 - u / upr) : u / upr;
-        const int k0 = (u % upr) * R.seg;
+        // unit: bra pair P, its primitive pairs [U.y, U.z) and the ket
+        // segment starting at U.w (k0 < the row's ket count)
+        const int4 U = R.units[u];
+        const int P = U.x, k0 = U.w;
         const int nk = R.tri ? P + 1 : R.nk;
-        if (k0 >= nk) continue;
         const int segn = min(R.seg, nk - k0);
+        const int bnp = U.z - U.y;   // bra primitive pairs of the unit (<= PSLICE)
         const ShellPair bra = A.pairs[R.boff + P];
         const int nrows = bra.same ? S::NA : S::NA + S::NB;
         double* rowA = rows;
         double* rowB = rows + (bra.same ? 0 : S::NA * N);
         for (int i = threadIdx.x; i < nrows * N; i += blockDim.x) rows[i] = 0.0;
-        stage_bra<LA, LB, LC, LD, T>(A, bra, bp);
+        stage_bra<LA, LB, LC, LD, T>(A, bra, U.y, bnp, bp);
         if (threadIdx.x == 0) {
             misc[1] = 0;   // survivors in the list
             misc[2] = 0;   // largest ket primitive-pair count in the list
         }
         __syncthreads();
-        const int nbs = (bra.npp + PSLICE - 1) / PSLICE;
         for (int t0 = 0;; t0 += (int)blockDim.x) {
             // screen the next kets and append the survivors
             if (t0 < segn) {
@@ -475,26 +475,31 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
             const int cnt0 = misc[1];
             const bool last = t0 + (int)blockDim.x >= segn;
             for (int b0 = 0; b0 < cnt0 && (cnt0 - b0 >= QB || last); b0 += QB) {
-                // digest the batch qlist[b0, b0+cnt): (64-quartet block, chunk, bra slice, ket slice)
+                // digest the batch qlist[b0, b0+cnt): a wavefront takes one
+                // chunk of bra components and 64 consecutive (ket slice,
+                // quartet) items, quartet index fastest, so that a short
+                // batch of heavily contracted kets still fills the lanes
                 const int cnt = min(QB, cnt0 - b0);
                 if (R.stat_nquart && threadIdx.x == 0) atomicAdd(R.stat_nquart, (unsigned long long)cnt);
-                const int nks = (misc[2] + PSLICE - 1) / PSLICE;
-                const int total = ((cnt + 63) / 64) * S::NCHUNK * nbs * nks;
+                // Classes that add partial integrals into the LDS buffer split
+                // the ket primitives into slices; the others take a quartet's
+                // whole ket contraction in one lane and digest it once.
+                const int nks = S::IBUF ? (misc[2] + PSLICE - 1) / PSLICE : 1;
+                const int nflat = cnt * nks;
+                const int total = ((nflat + 63) / 64) * S::NCHUNK;
                 for (int w = wv; w < total; w += nwv) {
                     const int chunk = w % S::NCHUNK;
-                    int r = w / S::NCHUNK;
-                    const int bs = r % nbs;
-                    r /= nbs;
-                    const int ks = r % nks;
-                    const int idx = (r / nks) * 64 + lane;
-                    if (idx >= cnt) continue;
+                    const int flat = (w / S::NCHUNK) * 64 + lane;
+                    if (flat >= nflat) continue;
+                    const int ks = flat / cnt;
+                    const int idx = flat - ks * cnt;
                     const int Q = qlist[b0 + idx];
                     const int2 qd = make_int2(R.boff + P, R.koff + Q);
                     const ShellPair ket = A.pairs[qd.y];
                     const int jq0 = ks * PSLICE;
                     if (jq0 >= ket.npp) continue;
-                    const int jq1 = min(ket.npp, jq0 + PSLICE);
-                    const int ip0 = bs * PSLICE, ip1 = min(bra.npp, ip0 + PSLICE);
+                    const int jq1 = S::IBUF ? min(ket.npp, jq0 + PSLICE) : ket.npp;
+                    const int ip0 = 0, ip1 = bnp;
                     if (chunk == 0) my_nprim += (unsigned long long)(ip1 - ip0) * (unsigned long long)(jq1 - jq0);
                     static_switch<0, S::NCHUNK>(chunk, [&](auto ic) {
                         constexpr int C = decltype(ic)::value;
@@ -663,7 +668,7 @@ inline size_t fock_class_lds_bytes(int N, int maxpp)
 {
     using S = ClassShape<LA, LB, LC, LD>;
     return sizeof(double) * ((size_t)(S::NA + S::NB) * N + S::IBUF_DOUBLES)
-         + sizeof(T) * (BOYS_TAYLOR * BOYS_NT + (size_t)maxpp * S::BPS)
+         + sizeof(T) * (BOYS_TAYLOR * BOYS_NT + (size_t)(maxpp < PSLICE ? maxpp : PSLICE) * S::BPS)
          + sizeof(int) * (ClassList<S::QB>::QLIST + 4);
 }
 

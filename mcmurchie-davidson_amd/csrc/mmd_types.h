@@ -46,11 +46,17 @@ struct ClassArgs {
 };
 
 // Row decomposition of one quartet class for the Fock kernels.
+// Primitive pairs per slice of the Fock-build class kernels: a work unit
+// takes at most PSLICE bra primitive pairs, a lane at most PSLICE ket ones.
+constexpr int PSLICE = 8;
+
 struct RowArgs {
     int boff, nb;               // bra pairs of the class: pairs[boff .. boff+nb)
     int koff, nk;               // ket pairs of the class
     int tri;                    // same pair class: ket index <= bra index
     int seg;                    // ket pairs per work unit
+    const int4* units;          // work units: {bra pair, first and end bra primitive pair, first ket}
+    int nunits;
     int maxpp;                  // largest primitive-pair count of the class's bra pairs
     int nshell;
     const double* Dsh;          // shell-block maxima of |dP| (nshell x nshell)

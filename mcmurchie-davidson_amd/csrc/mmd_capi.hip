@@ -382,7 +382,9 @@ int build_units(mmd_ctx* c)
         const bool tri = pcb == pck;
         const int64_t quartets = tri ? (int64_t)nb * (nb + 1) / 2 : (int64_t)nb * nk;
         // ket segment: long (survivors are batched inside a unit), but short
-        // enough for a few thousand work units per class
+        // enough for a few thousand segments per class.  (Shorter segments
+        // make a class that runs alone faster, but cost more than they gain
+        // when classes or molecules share the GPU.)
         int seg = 4096;
         while (seg > 256 && quartets / seg < 4096) seg /= 2;
         c->unit_seg[qc] = seg;

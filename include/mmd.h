@@ -167,10 +167,13 @@ int mmd_set_precision(mmd_ctx *ctx, double fp32_below);
 int mmd_fock_tiers(mmd_ctx *ctx, int64_t *nquart, int64_t *nquart_fp32);
 
 /*
- * Number of streams (1..4, default 4) the shell-quartet classes of a Fock
+ * Number of streams (1..8; default min(8, GPU_MAX_HW_QUEUES), 4 when that
+ * is unset) the shell-quartet classes of a Fock
  * build are spread over.  No counterpart in the reference: one stream suits
  * callers that run many contexts side by side (batches of small molecules),
- * where the other contexts supply the concurrency.
+ * where the other contexts supply the concurrency.  Classes on different
+ * streams run concurrently only as far as the process has hardware queues
+ * (HIP's GPU_MAX_HW_QUEUES).
  */
 int mmd_set_streams(mmd_ctx *ctx, int n);
 

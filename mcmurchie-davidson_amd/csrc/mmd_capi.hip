@@ -8,6 +8,7 @@
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 #include <algorithm>
 #include <string>
@@ -81,7 +82,7 @@ struct mmd_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // side streams for concurrent class kernels (default priority), with the
     // fork / join events that order them against st
-    static constexpr int NSIDE = 3;
+    static constexpr int NSIDE = 7;
     hipStream_t side[NSIDE] = {};
     hipEvent_t evfork = nullptr, evjoin[NSIDE] = {};
     int nstreams = 1 + NSIDE;           // streams the Fock classes are spread over
@@ -663,6 +664,14 @@ int mmd_ctx_create(mmd_ctx** out, int device_id)
     mmd_ctx* c = new (std::nothrow) mmd_ctx();
     if (!c) return fail(MMD_ERR_NOMEM, "context allocation failed");
     c->device = device_id;
+    // Classes on more streams than the process has hardware queues would
+    // share queues in an order the longest-first plan does not see: spread
+    // them over min(8, GPU_MAX_HW_QUEUES) streams (HIP's default is 4).
+    {
+        const char* hq = getenv("GPU_MAX_HW_QUEUES");
+        const int q = hq ? atoi(hq) : 4;
+        c->nstreams = std::max(1, std::min(q > 0 ? q : 4, 1 + mmd_ctx::NSIDE));
+    }
     int lds_max = 0;
     if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device_id) == hipSuccess &&
         lds_max > 0)

@@ -27,6 +27,11 @@ if not os.path.exists(LIB_PATH):
         "libmmd.so not found at %s; build it with `make -C mcmurchie-davidson_amd/csrc` "
         "(or __graft_entry__.build())" % LIB_PATH)
 
+# The Fock build runs its shell-quartet classes side by side on eight
+# streams; HIP maps a process's streams onto GPU_MAX_HW_QUEUES hardware
+# queues (four unless set), so ask for eight unless the process chose a
+# value (it takes effect if HIP has not been initialised yet).
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
 lib = ctypes.CDLL(LIB_PATH)
 
 _c_int = ctypes.c_int
@@ -294,7 +299,7 @@ class Context:
         check(lib.mmd_set_precision(self._h, float(fp32_below)))
 
     def set_streams(self, n):
-        """Streams (1..4) the Fock-build classes are spread over."""
+        """Streams (1..8) the Fock-build classes are spread over."""
         check(lib.mmd_set_streams(self._h, int(n)))
 
     # -- GPU-resident SCF loop -------------------------------------------------

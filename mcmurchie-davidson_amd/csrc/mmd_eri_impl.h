@@ -397,6 +397,7 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
 {
     using S = ClassShape<LA, LB, LC, LD>;
     constexpr int QB = S::QB;
+    static_assert(QB <= CLASS_BLOCK, "one thread per batch entry in the slice-count scan");
     extern __shared__ double dyn[];
     double* rows = dyn;
     double* ibuf = rows + (S::NA + S::NB) * A.N;
@@ -404,6 +405,8 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
     T* bp = lds_boys + BOYS_TAYLOR * BOYS_NT;
     int* qlist = reinterpret_cast<int*>(bp + (size_t)(R.maxpp < PSLICE ? R.maxpp : PSLICE) * S::BPS);
     int* misc = qlist + ClassList<QB>::QLIST;
+    int* pre = misc + 4;         // IBUF classes: prefix sums of the batch's ket slice counts
+    int* wsum = pre + QB + 1;    // per-wavefront partial sums of that scan
     if constexpr (std::is_same<T, float>::value)
         boys_stage_lds<S::L, float>(lds_boys, A.boysf);
     else
@@ -441,17 +444,13 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
         double* rowB = rows + (bra.same ? 0 : S::NA * N);
         for (int i = threadIdx.x; i < nrows * N; i += blockDim.x) rows[i] = 0.0;
         stage_bra<LA, LB, LC, LD, T>(A, bra, U.y, bnp, bp);
-        if (threadIdx.x == 0) {
-            misc[1] = 0;   // survivors in the list
-            misc[2] = 0;   // largest ket primitive-pair count in the list
-        }
+        if (threadIdx.x == 0) misc[1] = 0;   // survivors in the list
         __syncthreads();
         for (int t0 = 0;; t0 += (int)blockDim.x) {
             // screen the next kets and append the survivors
             if (t0 < segn) {
                 const int t = t0 + threadIdx.x;
                 bool keep = false;
-                int npk = 0;
                 if (t < segn) {
                     const ShellPair ket = A.pairs[R.koff + k0 + t];
                     const double* Dsh = R.Dsh;
@@ -460,39 +459,62 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
                                            fmax(Dsh[bra.sb * ns + ket.sa], Dsh[bra.sb * ns + ket.sb]));
                     const double bound = bra.q * ket.q * dm;
                     keep = bound >= A.tol && bound >= R.tier_lo && bound < R.tier_hi;
-                    npk = ket.npp;
                 }
                 const unsigned long long m = __ballot(keep);
                 int pos = 0;
                 if (lane == 0 && m) pos = atomicAdd(&misc[1], __popcll(m));
                 pos = __shfl(pos, 0);
-                if (keep) {
-                    qlist[pos + __popcll(m & ((1ull << lane) - 1ull))] = k0 + t;
-                    atomicMax(&misc[2], npk);
-                }
+                if (keep) qlist[pos + __popcll(m & ((1ull << lane) - 1ull))] = k0 + t;
             }
             __syncthreads();
             const int cnt0 = misc[1];
             const bool last = t0 + (int)blockDim.x >= segn;
             for (int b0 = 0; b0 < cnt0 && (cnt0 - b0 >= QB || last); b0 += QB) {
                 // digest the batch qlist[b0, b0+cnt): a wavefront takes one
-                // chunk of bra components and 64 consecutive (ket slice,
-                // quartet) items, quartet index fastest, so that a short
-                // batch of heavily contracted kets still fills the lanes
+                // chunk of bra components and 64 consecutive lane items.
+                // Classes that add partial integrals into the LDS buffer
+                // split each ket's primitives into slices of PSLICE; their
+                // lane items are the (quartet, ket slice) pairs that exist,
+                // numbered through the prefix sums pre[] of the quartets'
+                // slice counts, so that lanes stay busy however the batch
+                // mixes contraction lengths.  The other classes take a
+                // quartet's whole ket contraction in one lane and digest it.
                 const int cnt = min(QB, cnt0 - b0);
                 if (R.stat_nquart && threadIdx.x == 0) atomicAdd(R.stat_nquart, (unsigned long long)cnt);
-                // Classes that add partial integrals into the LDS buffer split
-                // the ket primitives into slices; the others take a quartet's
-                // whole ket contraction in one lane and digest it once.
-                const int nks = S::IBUF ? (misc[2] + PSLICE - 1) / PSLICE : 1;
-                const int nflat = cnt * nks;
+                int nflat = cnt;
+                if constexpr (S::IBUF) {
+                    const int j = threadIdx.x;   // cnt <= QB <= CLASS_BLOCK
+                    int x = 0;
+                    if (j < cnt) x = (A.pairs[R.koff + qlist[b0 + j]].npp + PSLICE - 1) / PSLICE;
+                    for (int off = 1; off < 64; off <<= 1) {
+                        const int y = __shfl_up(x, off);
+                        if (lane >= off) x += y;
+                    }
+                    if (lane == 63) wsum[wv] = x;
+                    __syncthreads();
+                    for (int k = 0; k < wv; ++k) x += wsum[k];
+                    if (j < cnt) pre[j + 1] = x;
+                    if (j == 0) pre[0] = 0;
+                    __syncthreads();
+                    nflat = pre[cnt];
+                }
                 const int total = ((nflat + 63) / 64) * S::NCHUNK;
                 for (int w = wv; w < total; w += nwv) {
                     const int chunk = w % S::NCHUNK;
                     const int flat = (w / S::NCHUNK) * 64 + lane;
                     if (flat >= nflat) continue;
-                    const int ks = flat / cnt;
-                    const int idx = flat - ks * cnt;
+                    int idx = flat, ks = 0;
+                    if constexpr (S::IBUF) {
+                        // the quartet whose slices cover item flat: pre[idx] <= flat < pre[idx + 1]
+                        int lo = 0, hi = cnt - 1;
+                        while (lo < hi) {
+                            const int mid = (lo + hi + 1) >> 1;
+                            if (pre[mid] <= flat) lo = mid;
+                            else hi = mid - 1;
+                        }
+                        idx = lo;
+                        ks = flat - pre[lo];
+                    }
                     const int Q = qlist[b0 + idx];
                     const int2 qd = make_int2(R.boff + P, R.koff + Q);
                     const ShellPair ket = A.pairs[qd.y];
@@ -669,7 +691,7 @@ inline size_t fock_class_lds_bytes(int N, int maxpp)
     using S = ClassShape<LA, LB, LC, LD>;
     return sizeof(double) * ((size_t)(S::NA + S::NB) * N + S::IBUF_DOUBLES)
          + sizeof(T) * (BOYS_TAYLOR * BOYS_NT + (size_t)(maxpp < PSLICE ? maxpp : PSLICE) * S::BPS)
-         + sizeof(int) * (ClassList<S::QB>::QLIST + 4);
+         + sizeof(int) * (ClassList<S::QB>::QLIST + 4 + S::QB + 1 + CLASS_BLOCK / 64);
 }
 
 }  // namespace mmd

@@ -136,6 +136,12 @@ struct mmd_ctx {
     // per-class statistics of the last symmetrised Fock build
     int profiling = 0;                  // 0 off; 1 events, classes concurrent; 2 events, classes serial
     hipEvent_t evc[2 * NQCLASS] = {};
+    // Stream plan of the Fock build: class durations measured with the
+    // per-class events in one build out of PLAN_EVERY (0: not measured yet).
+    static constexpr int PLAN_EVERY = 16;
+    double plan_ms[NQCLASS] = {0};
+    int plan_countdown = 0;
+    bool plan_measuring = false;
     DevBuf<unsigned long long> d_nprim;   // [0,21): primitive quartets, [21,42): shell quartets
     DevBuf<int> d_rowctr;
     // Work units of the row-kernel classes (built once per pair table):
@@ -359,6 +365,8 @@ int build_pairs(mmd_ctx* c, const std::vector<double>* qpair)
     HIP_TRY(c->d_count.reserve(NQCLASS));
     c->pairs_ok = true;
     c->units_ok = false;
+    for (int qc = 0; qc < NQCLASS; ++qc) c->plan_ms[qc] = 0.0;
+    c->plan_countdown = 0;
     return MMD_OK;
 }
 
@@ -493,8 +501,20 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol)
         cost[qc] = prim * (40.0 + 4.0 * nherm(L) + (double)ncart(la) * ncart(lb) * ncart(lc) * ncart(ld));
         order[nord++] = qc;
     }
-    std::sort(order, order + nord, [&](int x, int y) { return cost[x] > cost[y]; });
     const int nstream = c->profiling == 2 ? 1 : c->nstreams;
+    // Plan on the measured class durations once they exist: the model above
+    // misses the latency floor of the small classes (150-300 us each at
+    // (H2O)8/cc-pVDZ, whatever their work), which left the longest stream
+    // twice as long as the others.  One build in PLAN_EVERY measures again.
+    const bool measure = c->profiling == 0 && nstream > 1 && c->plan_countdown <= 0;
+    bool measured = nord > 0;
+    for (int oi = 0; oi < nord; ++oi) measured = measured && c->plan_ms[order[oi]] > 0.0;
+    if (measured)
+        for (int oi = 0; oi < nord; ++oi) cost[order[oi]] = c->plan_ms[order[oi]];
+    if (measure && !c->evc[0])
+        for (int i = 0; i < 2 * NQCLASS; ++i) HIP_TRY(hipEventCreate(&c->evc[i]));
+    c->plan_measuring = measure;
+    std::sort(order, order + nord, [&](int x, int y) { return cost[x] > cost[y]; });
     hipStream_t streams[1 + mmd_ctx::NSIDE] = {c->st};
     for (int i = 0; i < mmd_ctx::NSIDE; ++i) streams[1 + i] = c->side[i];
     double load[1 + mmd_ctx::NSIDE] = {0};
@@ -516,7 +536,7 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol)
         qclass_decode(qc, &la, &lb, &lc, &ld);
         const int nb = c->pc_off[pcb + 1] - c->pc_off[pcb];
         const int nk = c->pc_off[pck + 1] - c->pc_off[pck];
-        if (c->profiling) HIP_TRY(hipEventRecord(c->evc[2 * qc], sq));
+        if (c->profiling || measure) HIP_TRY(hipEventRecord(c->evc[2 * qc], sq));
         if (row_kernel[qc]) {
             RowArgs R;
             R.boff = c->pc_off[pcb];
@@ -533,7 +553,7 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol)
             const int64_t units = c->unit_off[qc + 1] - c->unit_off[qc];
             R.nunits = (int)units;
             if (units == 0) {
-                if (c->profiling) HIP_TRY(hipEventRecord(c->evc[2 * qc + 1], sq));
+                if (c->profiling || measure) HIP_TRY(hipEventRecord(c->evc[2 * qc + 1], sq));
                 continue;
             }
             R.stat_nquart = c->d_nprim.p + NQCLASS + qc;
@@ -561,7 +581,7 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol)
                 HIP_TRY(launch_count_prims(W.quart, W.count, c->d_pairs.p, c->d_nprim.p + qc, sq));
             HIP_TRY(launch_fock_wave_class(la, lb, lc, ld, W, c->qc_total[qc], sq));
         }
-        if (c->profiling) HIP_TRY(hipEventRecord(c->evc[2 * qc + 1], sq));
+        if (c->profiling || measure) HIP_TRY(hipEventRecord(c->evc[2 * qc + 1], sq));
     }
     for (int i = 1; i < nstream; ++i) {
         if (!used[i]) continue;
@@ -1005,6 +1025,17 @@ int mmd_fock_delta(mmd_ctx* ctx, const double* dP_re, const double* dP_im, doubl
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->last_fock_ms = ms;
+    if (!raw && ctx->plan_measuring) {
+        for (int qc = 0; qc < NQCLASS; ++qc) {
+            float cms = 0.f;
+            if (ctx->qc_total[qc]) HIP_TRY(hipEventElapsedTime(&cms, ctx->evc[2 * qc], ctx->evc[2 * qc + 1]));
+            ctx->plan_ms[qc] = cms > 0.f ? cms : 1e-3;
+        }
+        ctx->plan_measuring = false;
+        ctx->plan_countdown = mmd_ctx::PLAN_EVERY;
+    } else if (!raw) {
+        --ctx->plan_countdown;
+    }
     if (ctx->profiling && !raw) return collect_profile(ctx);
     return MMD_OK;
 }

@@ -65,8 +65,13 @@ struct ClassShape {
     // and digest each quartet once; QB is their batch size (buffer <= 16 KB).
     static constexpr int NABCD = NAB * NCD;
     static constexpr bool IBUF = NABCD <= 32;
-    static constexpr int QB = IBUF ? (2048 / NABCD >= 256 ? 256 : (2048 / NABCD >= 128 ? 128 : 64)) : 256;
-    static constexpr int IBUF_DOUBLES = IBUF ? QB * NABCD : 0;
+    // The other classes digest each (quartet, chunk) item; when a quartet
+    // has several chunks, their ket-side Coulomb blocks are summed in an LDS
+    // buffer per batch (KJ) and added to G once per quartet.
+    static constexpr bool KJ = !IBUF && NCHUNK > 1;
+    static constexpr int QB = IBUF ? (2048 / NABCD >= 256 ? 256 : (2048 / NABCD >= 128 ? 128 : 64))
+                                   : (KJ ? (2048 / NCD >= 256 ? 256 : (2048 / NCD >= 128 ? 128 : 64)) : 256);
+    static constexpr int IBUF_DOUBLES = IBUF ? QB * NABCD : (KJ ? QB * NCD : 0);
 };
 
 __device__ __forceinline__ double dmax6(double a, double b, double c, double d, double e, double f)
@@ -210,9 +215,10 @@ __device__ __forceinline__ void quartet_compute(const ClassArgs& A, const T* lds
 // Digest the integrals acc of a quartet (bra components A0..A1-1) into G.
 // rowA / rowB: rows of G for the bra shells' functions (row x of shell a at
 // rowA + x*N) accumulated in LDS; the ket-side Coulomb block goes to G.
-template <int LA, int LB, int LC, int LD, int A0, int A1>
+template <int LA, int LB, int LC, int LD, int A0, int A1, bool KJ = false>
 __device__ __forceinline__ void quartet_digest(const ClassArgs& A, int2 qd, const ShellPair& bra,
-                                               const ShellPair& ket, double* acc, double* rowA, double* rowB)
+                                               const ShellPair& ket, double* acc, double* rowA, double* rowB,
+                                               double* kj = nullptr)
 {
     using S = ClassShape<LA, LB, LC, LD>;
     constexpr int NCH = A1 - A0;
@@ -267,7 +273,8 @@ __device__ __forceinline__ void quartet_digest(const ClassArgs& A, int2 qd, cons
         });
         atomicAdd(&rowA[(size_t)a * N + bfb + b], s);
     });
-    // Coulomb: G_cd += sum_ab (ab|cd) D_ab   (ket block, global)
+    // Coulomb: G_cd += sum_ab (ab|cd) D_ab   (ket block: to G, or with KJ
+    // to the quartet's LDS row kj, added to G after the batch)
     sfor<NCD>([&](auto icd) {
         constexpr int cd = decltype(icd)::value;
         constexpr int c = cd / S::ND, d = cd % S::ND;
@@ -277,7 +284,10 @@ __device__ __forceinline__ void quartet_digest(const ClassArgs& A, int2 qd, cons
             constexpr int a = ab / S::NB, b = ab % S::NB;
             s = fma(acc[AI(ab, cd)], D[(size_t)(bfa + a) * N + bfb + b], s);
         });
-        atomicAdd(&G[(size_t)(bfc + c) * N + bfd + d], s);
+        if constexpr (KJ)
+            atomicAdd(&kj[cd], s);
+        else
+            atomicAdd(&G[(size_t)(bfc + c) * N + bfd + d], s);
     });
     // Exchange (cython/fock.pyx:82-85): G_bd -= 1/4 (ab|cd) D_ac ;
     // G_bc -= 1/4 (ab|cd) D_ad ; G_ac -= 1/4 (ab|cd) D_bd ; G_ad -= 1/4 (ab|cd) D_bc.
@@ -411,7 +421,7 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
         boys_stage_lds<S::L, float>(lds_boys, A.boysf);
     else
         boys_stage_lds<S::L, double>(lds_boys, A.boys);
-    if constexpr (S::IBUF)
+    if constexpr (S::IBUF_DOUBLES > 0)
         for (int i = threadIdx.x; i < S::IBUF_DOUBLES; i += blockDim.x) ibuf[i] = 0.0;
     const int N = A.N;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -535,9 +545,22 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
                         } else {
                             double accd[(A1 - A0) * S::NCD];
                             sfor<(A1 - A0) * S::NCD>([&](auto k) { accd[k] = (double)acc[k]; });
-                            quartet_digest<LA, LB, LC, LD, A0, A1>(A, qd, bra, ket, accd, rowA, rowB);
+                            quartet_digest<LA, LB, LC, LD, A0, A1, S::KJ>(A, qd, bra, ket, accd, rowA, rowB,
+                                                                          ibuf + (size_t)idx * S::NCD);
                         }
                     });
+                }
+                if constexpr (S::KJ) {
+                    // the summed ket-side Coulomb blocks of the batch into G
+                    __syncthreads();
+                    for (int i = threadIdx.x; i < cnt * S::NCD; i += blockDim.x) {
+                        const double v = ibuf[i];
+                        if (v == 0.0) continue;
+                        ibuf[i] = 0.0;
+                        const int idx = i / S::NCD, cd = i - idx * S::NCD;
+                        const ShellPair ket = A.pairs[R.koff + qlist[b0 + idx]];
+                        atomicAdd(&A.G[(size_t)(ket.bfa + cd / S::ND) * N + ket.bfb + cd % S::ND], v);
+                    }
                 }
                 if constexpr (S::IBUF) {
                     // every quartet of the batch once: its summed integrals into G

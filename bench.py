@@ -38,11 +38,11 @@ FP64_PEAK_TFLOPS = 78.6
 # counters read for a copy of known size (recorded with the command lines in
 # profiles/); a class absent here reports null.
 TRAFFIC = {
-    (0, 0, 0, 0): 2.243e+07, (1, 0, 0, 0): 4.229e+07, (1, 0, 1, 0): 4.496e+07, (1, 1, 0, 0): 2.076e+07,
-    (1, 1, 1, 0): 5.515e+07, (1, 1, 1, 1): 5.183e+07, (2, 0, 0, 0): 1.934e+07, (2, 0, 1, 0): 3.839e+07,
-    (2, 0, 1, 1): 3.261e+07, (2, 0, 2, 0): 9.423e+06, (2, 1, 0, 0): 1.238e+07, (2, 1, 1, 0): 3.853e+07,
-    (2, 1, 1, 1): 3.987e+07, (2, 1, 2, 0): 1.935e+07, (2, 1, 2, 1): 2.869e+07, (2, 2, 0, 0): 3.702e+06,
-    (2, 2, 1, 0): 1.155e+07, (2, 2, 1, 1): 1.328e+07, (2, 2, 2, 0): 7.305e+06, (2, 2, 2, 1): 2.035e+07,
+    (0, 0, 0, 0): 2.242e+07, (1, 0, 0, 0): 4.229e+07, (1, 0, 1, 0): 4.498e+07, (1, 1, 0, 0): 2.071e+07,
+    (1, 1, 1, 0): 5.514e+07, (1, 1, 1, 1): 7.443e+06, (2, 0, 0, 0): 1.912e+07, (2, 0, 1, 0): 3.826e+07,
+    (2, 0, 1, 1): 1.153e+07, (2, 0, 2, 0): 5.899e+06, (2, 1, 0, 0): 1.243e+07, (2, 1, 1, 0): 1.576e+07,
+    (2, 1, 1, 1): 7.758e+06, (2, 1, 2, 0): 5.656e+06, (2, 1, 2, 1): 5.973e+06, (2, 2, 0, 0): 3.456e+06,
+    (2, 2, 1, 0): 5.487e+06, (2, 2, 1, 1): 2.828e+06, (2, 2, 2, 0): 2.533e+06, (2, 2, 2, 1): 5.283e+06,
 }
 
 

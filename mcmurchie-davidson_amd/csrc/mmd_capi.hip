@@ -470,7 +470,7 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol)
         const int rc = build_units(c);
         if (rc) return rc;
     }
-    bool row_kernel[NQCLASS];
+    bool row_kernel[NQCLASS], kj_lds[NQCLASS];
     int maxpp[NQCLASS];
     for (int qc = 0; qc < NQCLASS; ++qc) {
         int la, lb, lc, ld, pcb, pck;
@@ -478,7 +478,11 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol)
         class_of(qc, &pcb, &pck);
         maxpp[qc] = 1;
         for (int P = c->pc_off[pcb]; P < c->pc_off[pcb + 1]; ++P) maxpp[qc] = std::max(maxpp[qc], c->pairs[P].npp);
-        row_kernel[qc] = !fock_class_is_wave(la, lb, lc, ld) && fock_class_lds(qc, 0, c->nbf, maxpp[qc]) <= c->max_lds;
+        // the ket-side Coulomb buffer where it fits; the row kernel without it
+        // where only that fits; else the wavefront-per-quartet kernel
+        kj_lds[qc] = fock_class_lds(qc, 0, c->nbf, maxpp[qc], 1) <= c->max_lds;
+        row_kernel[qc] = !fock_class_is_wave(la, lb, lc, ld) &&
+                         (kj_lds[qc] || fock_class_lds(qc, 0, c->nbf, maxpp[qc], 0) <= c->max_lds);
     }
     // Classes run concurrently on the main and side streams (longest first,
     // each to the least loaded stream), so that the small classes fill the
@@ -548,6 +552,7 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol)
             R.Dsh = c->d_Dsh.p;
             R.row_counter = c->d_rowctr.p + qc;
             R.maxpp = maxpp[qc];
+            R.kj = kj_lds[qc] ? 1 : 0;
             R.seg = c->unit_seg[qc];
             R.units = c->d_units.p + c->unit_off[qc];
             const int64_t units = c->unit_off[qc + 1] - c->unit_off[qc];

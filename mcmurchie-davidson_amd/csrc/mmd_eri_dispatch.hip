@@ -13,8 +13,8 @@ namespace mmd {
 #define DECL(a, b, c, d)                                                                                         \
     hipError_t fock_class_launch_##a##b##c##d##_d(const ClassArgs& A, const RowArgs& R, int grid, hipStream_t st); \
     hipError_t fock_class_launch_##a##b##c##d##_f(const ClassArgs& A, const RowArgs& R, int grid, hipStream_t st); \
-    size_t fock_class_lds_##a##b##c##d##_d(int N, int maxpp);                                                    \
-    size_t fock_class_lds_##a##b##c##d##_f(int N, int maxpp);
+    size_t fock_class_lds_##a##b##c##d##_d(int N, int maxpp, int kj);                                            \
+    size_t fock_class_lds_##a##b##c##d##_f(int N, int maxpp, int kj);
 MMD_CLASSES(DECL)
 #undef DECL
 
@@ -62,13 +62,13 @@ hipError_t launch_fock_class(int qc, int fp32, const ClassArgs& A, const RowArgs
     return hipErrorInvalidValue;  // wavefront-per-quartet classes are launched separately
 }
 
-size_t fock_class_lds(int qc, int fp32, int N, int maxpp)
+size_t fock_class_lds(int qc, int fp32, int N, int maxpp, int kj)
 {
     int la, lb, lc, ld;
     qclass_decode(qc, &la, &lb, &lc, &ld);
 #define CASE(a, b, c, d)                          \
     if (la == a && lb == b && lc == c && ld == d) \
-        return fp32 ? fock_class_lds_##a##b##c##d##_f(N, maxpp) : fock_class_lds_##a##b##c##d##_d(N, maxpp);
+        return fp32 ? fock_class_lds_##a##b##c##d##_f(N, maxpp, kj) : fock_class_lds_##a##b##c##d##_d(N, maxpp, kj);
     MMD_CLASSES(CASE)
 #undef CASE
     return 0;

@@ -71,7 +71,8 @@ struct ClassShape {
     static constexpr bool KJ = !IBUF && NCHUNK > 1;
     static constexpr int QB = IBUF ? (2048 / NABCD >= 256 ? 256 : (2048 / NABCD >= 128 ? 128 : 64))
                                    : (KJ ? (2048 / NCD >= 256 ? 256 : (2048 / NCD >= 128 ? 128 : 64)) : 256);
-    static constexpr int IBUF_DOUBLES = IBUF ? QB * NABCD : (KJ ? QB * NCD : 0);
+    static constexpr int IBUF_DOUBLES = IBUF ? QB * NABCD : 0;
+    static constexpr int KJ_DOUBLES = KJ ? QB * NCD : 0;   // used when the launch has LDS for it
 };
 
 __device__ __forceinline__ double dmax6(double a, double b, double c, double d, double e, double f)
@@ -215,7 +216,7 @@ __device__ __forceinline__ void quartet_compute(const ClassArgs& A, const T* lds
 // Digest the integrals acc of a quartet (bra components A0..A1-1) into G.
 // rowA / rowB: rows of G for the bra shells' functions (row x of shell a at
 // rowA + x*N) accumulated in LDS; the ket-side Coulomb block goes to G.
-template <int LA, int LB, int LC, int LD, int A0, int A1, bool KJ = false>
+template <int LA, int LB, int LC, int LD, int A0, int A1>
 __device__ __forceinline__ void quartet_digest(const ClassArgs& A, int2 qd, const ShellPair& bra,
                                                const ShellPair& ket, double* acc, double* rowA, double* rowB,
                                                double* kj = nullptr)
@@ -273,8 +274,8 @@ __device__ __forceinline__ void quartet_digest(const ClassArgs& A, int2 qd, cons
         });
         atomicAdd(&rowA[(size_t)a * N + bfb + b], s);
     });
-    // Coulomb: G_cd += sum_ab (ab|cd) D_ab   (ket block: to G, or with KJ
-    // to the quartet's LDS row kj, added to G after the batch)
+    // Coulomb: G_cd += sum_ab (ab|cd) D_ab   (ket block: to G, or to the
+    // quartet's LDS row kj when given, added to G after the batch)
     sfor<NCD>([&](auto icd) {
         constexpr int cd = decltype(icd)::value;
         constexpr int c = cd / S::ND, d = cd % S::ND;
@@ -284,7 +285,7 @@ __device__ __forceinline__ void quartet_digest(const ClassArgs& A, int2 qd, cons
             constexpr int a = ab / S::NB, b = ab % S::NB;
             s = fma(acc[AI(ab, cd)], D[(size_t)(bfa + a) * N + bfb + b], s);
         });
-        if constexpr (KJ)
+        if (kj)
             atomicAdd(&kj[cd], s);
         else
             atomicAdd(&G[(size_t)(bfc + c) * N + bfd + d], s);
@@ -411,7 +412,10 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
     extern __shared__ double dyn[];
     double* rows = dyn;
     double* ibuf = rows + (S::NA + S::NB) * A.N;
-    T* lds_boys = reinterpret_cast<T*>(ibuf + S::IBUF_DOUBLES);
+    // ket-side Coulomb buffer (KJ classes, when the launch has room for it)
+    const bool kj = S::KJ && R.kj;
+    const int nbuf = S::IBUF_DOUBLES + (kj ? S::KJ_DOUBLES : 0);
+    T* lds_boys = reinterpret_cast<T*>(ibuf + nbuf);
     T* bp = lds_boys + BOYS_TAYLOR * BOYS_NT;
     int* qlist = reinterpret_cast<int*>(bp + (size_t)(R.maxpp < PSLICE ? R.maxpp : PSLICE) * S::BPS);
     int* misc = qlist + ClassList<QB>::QLIST;
@@ -421,8 +425,7 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
         boys_stage_lds<S::L, float>(lds_boys, A.boysf);
     else
         boys_stage_lds<S::L, double>(lds_boys, A.boys);
-    if constexpr (S::IBUF_DOUBLES > 0)
-        for (int i = threadIdx.x; i < S::IBUF_DOUBLES; i += blockDim.x) ibuf[i] = 0.0;
+    for (int i = threadIdx.x; i < nbuf; i += blockDim.x) ibuf[i] = 0.0;
     const int N = A.N;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int nwv = (int)(blockDim.x >> 6);
@@ -545,13 +548,14 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
                         } else {
                             double accd[(A1 - A0) * S::NCD];
                             sfor<(A1 - A0) * S::NCD>([&](auto k) { accd[k] = (double)acc[k]; });
-                            quartet_digest<LA, LB, LC, LD, A0, A1, S::KJ>(A, qd, bra, ket, accd, rowA, rowB,
-                                                                          ibuf + (size_t)idx * S::NCD);
+                            quartet_digest<LA, LB, LC, LD, A0, A1>(A, qd, bra, ket, accd, rowA, rowB,
+                                                                   kj ? ibuf + (size_t)idx * S::NCD : nullptr);
                         }
                     });
                 }
-                if constexpr (S::KJ) {
+                if (kj) {
                     // the summed ket-side Coulomb blocks of the batch into G
+                    // (kj is uniform across the workgroup)
                     __syncthreads();
                     for (int i = threadIdx.x; i < cnt * S::NCD; i += blockDim.x) {
                         const double v = ibuf[i];
@@ -709,10 +713,10 @@ __global__ void __launch_bounds__(256) eri_tensor_class_kernel(ClassArgs A, int 
 }
 
 template <int LA, int LB, int LC, int LD, typename T>
-inline size_t fock_class_lds_bytes(int N, int maxpp)
+inline size_t fock_class_lds_bytes(int N, int maxpp, int kj)
 {
     using S = ClassShape<LA, LB, LC, LD>;
-    return sizeof(double) * ((size_t)(S::NA + S::NB) * N + S::IBUF_DOUBLES)
+    return sizeof(double) * ((size_t)(S::NA + S::NB) * N + S::IBUF_DOUBLES + (kj ? S::KJ_DOUBLES : 0))
          + sizeof(T) * (BOYS_TAYLOR * BOYS_NT + (size_t)(maxpp < PSLICE ? maxpp : PSLICE) * S::BPS)
          + sizeof(int) * (ClassList<S::QB>::QLIST + 4 + S::QB + 1 + CLASS_BLOCK / 64);
 }

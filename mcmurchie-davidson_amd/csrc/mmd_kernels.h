@@ -24,7 +24,8 @@ inline __host__ __device__ void qclass_decode(int qc, int* la, int* lb, int* lc,
 
 // fp32 = 1 selects the FP32-tier instantiation of the class kernel.
 hipError_t launch_fock_class(int qc, int fp32, const ClassArgs& A, const RowArgs& R, int grid, hipStream_t st);
-size_t fock_class_lds(int qc, int fp32, int N, int maxpp);
+// kj = 1: including the ket-side Coulomb buffer of the classes that have one.
+size_t fock_class_lds(int qc, int fp32, int N, int maxpp, int kj);
 hipError_t launch_fock_wave_class(int la, int lb, int lc, int ld, const ClassArgs& A, int64_t max_quartets,
                                   hipStream_t st);
 // Stored ERI tensor from shell quartets: class kernels (every class but

@@ -58,6 +58,7 @@ struct RowArgs {
     const int4* units;          // work units: {bra pair, first and end bra primitive pair, first ket}
     int nunits;
     int maxpp;                  // largest primitive-pair count of the class's bra pairs
+    int kj;                     // LDS holds the ket-side Coulomb buffer (classes that have one)
     int nshell;
     const double* Dsh;          // shell-block maxima of |dP| (nshell x nshell)
     double tier_lo, tier_hi;    // this launch takes quartets with tier_lo <= bound < tier_hi

@@ -38,11 +38,11 @@ FP64_PEAK_TFLOPS = 78.6
 # counters read for a copy of known size (recorded with the command lines in
 # profiles/); a class absent here reports null.
 TRAFFIC = {
-    (0, 0, 0, 0): 2.242e+07, (1, 0, 0, 0): 4.229e+07, (1, 0, 1, 0): 4.498e+07, (1, 1, 0, 0): 2.071e+07,
-    (1, 1, 1, 0): 5.514e+07, (1, 1, 1, 1): 7.443e+06, (2, 0, 0, 0): 1.912e+07, (2, 0, 1, 0): 3.826e+07,
-    (2, 0, 1, 1): 1.153e+07, (2, 0, 2, 0): 5.899e+06, (2, 1, 0, 0): 1.243e+07, (2, 1, 1, 0): 1.576e+07,
-    (2, 1, 1, 1): 7.758e+06, (2, 1, 2, 0): 5.656e+06, (2, 1, 2, 1): 5.973e+06, (2, 2, 0, 0): 3.456e+06,
-    (2, 2, 1, 0): 5.487e+06, (2, 2, 1, 1): 2.828e+06, (2, 2, 2, 0): 2.533e+06, (2, 2, 2, 1): 5.283e+06,
+    (0, 0, 0, 0): 2.243e+07, (1, 0, 0, 0): 4.228e+07, (1, 0, 1, 0): 4.499e+07, (1, 1, 0, 0): 2.057e+07,
+    (1, 1, 1, 0): 5.509e+07, (1, 1, 1, 1): 7.417e+06, (2, 0, 0, 0): 1.914e+07, (2, 0, 1, 0): 3.835e+07,
+    (2, 0, 1, 1): 1.156e+07, (2, 0, 2, 0): 5.956e+06, (2, 1, 0, 0): 1.250e+07, (2, 1, 1, 0): 1.574e+07,
+    (2, 1, 1, 1): 7.839e+06, (2, 1, 2, 0): 5.723e+06, (2, 1, 2, 1): 6.062e+06, (2, 2, 0, 0): 3.400e+06,
+    (2, 2, 1, 0): 5.654e+06, (2, 2, 1, 1): 2.865e+06, (2, 2, 2, 0): 2.498e+06, (2, 2, 2, 1): 5.490e+06,
 }
 
 

@@ -114,6 +114,7 @@ struct mmd_ctx {
     int pc_off[NPAIRCLASS + 1] = {0};
     DevBuf<ShellPair> d_pairs;
     DevBuf<PrimPair> d_pp;
+    DevBuf<double> d_onee;   // S, T, V, M (3), L (3) of mmd_one_electron
     int64_t npp_total = 0;
 
     // Schwarz
@@ -832,7 +833,7 @@ int mmd_one_electron(mmd_ctx* ctx, const double center[3], double* S, double* T,
     if (ctx->nbf == 0) return fail(MMD_ERR_STATE, "no basis set");
     if (!center || !S || !T || !V || !M || !L) return fail(MMD_ERR_ARG, "null pointer");
     const size_t nn = (size_t)ctx->nbf * ctx->nbf;
-    DevBuf<double> d;
+    DevBuf<double>& d = ctx->d_onee;   // kept between calls (no allocation per SCF)
     HIP_TRY(d.reserve(9 * nn));
     HIP_TRY(launch_one_electron(ctx->nbf, ctx->d_bf.p, ctx->d_bf_exp.p, ctx->d_bf_w.p, ctx->natom,
                                 ctx->natom ? ctx->d_atoms.p : nullptr, center[0], center[1], center[2], ctx->d_boys.p,

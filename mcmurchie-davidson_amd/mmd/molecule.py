@@ -214,8 +214,9 @@ class Molecule(SCF):
         ERI tensor (conventional) (mmd/molecule.py:90-102)."""
         self.one_electron_integrals()
         if direct:
-            diag = self.native.schwarz()
+            diag = self.native.schwarz()   # also installs the screen in the context
             self.screen = ScreenDict(diag)
+            self.native._screen_key = (id(self.screen), len(self.screen))
         else:
             self.two_electron_integrals()
         self.is_built = True

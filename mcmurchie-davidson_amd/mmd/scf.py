@@ -11,6 +11,7 @@ import contextlib
 
 import numpy as np
 import scipy.linalg
+import scipy.linalg.lapack
 
 from mmd.integrals.fock import formPT
 
@@ -38,6 +39,28 @@ def _thread_plan(N):
     is no faster on more threads at N <= 400 (and slower at N ~ 200), so it
     runs on one."""
     return (1 if N < 128 else 4), 1
+
+
+_EVD_LWORK = {}
+
+
+def _eigh(A):
+    """Eigenvalues (ascending) and eigenvectors of a symmetric matrix by
+    LAPACK's divide and conquer, as scipy.linalg.eigh(A, driver="evd")
+    computes them (same routine, lower triangle, same results), without its
+    argument checks and workspace query on every call; complex input goes to
+    scipy.linalg.eigh."""
+    A = np.asarray(A)
+    if A.dtype != np.float64 or A.ndim != 2:
+        return scipy.linalg.eigh(A, driver="evd")
+    n = A.shape[0]
+    lw = _EVD_LWORK.get(n)
+    if lw is None:
+        lw = _EVD_LWORK[n] = scipy.linalg.lapack.dsyevd_lwork(n, compute_v=1, lower=1)[:2]
+    w, v, info = scipy.linalg.lapack.dsyevd(A, compute_v=1, lower=1, lwork=int(lw[0]), liwork=int(lw[1]))
+    if info != 0:
+        raise np.linalg.LinAlgError("dsyevd failed (info = %d)" % info)
+    return w, v
 
 
 def _re(A):
@@ -126,7 +149,7 @@ class SCF(object):
                 self.orthoFock()
 
             with _blas_threads(getattr(self, "_eigh_threads", None)):
-                E, self.CO = scipy.linalg.eigh(_re(self.FO), driver="evd")
+                E, self.CO = _eigh(_re(self.FO))
             C = np.dot(self.X, self.CO)
             self.C = C
             self.MO = E
@@ -185,7 +208,7 @@ class SCF(object):
             else:
                 ctx.scf_step(self.scrTol, False, False, FO)
             with _blas_threads(getattr(self, "_eigh_threads", None)):
-                E, CO = scipy.linalg.eigh(FO, driver="evd")
+                E, CO = _eigh(FO)
             e_el, rms = ctx.scf_density(CO)
             self.el_energy = np.complex128(e_el)
             self.energy = self.el_energy + self.nuc_energy

@@ -47,6 +47,7 @@ __host__ __device__ constexpr int chunk_size(int nab, int hcd, int ncd, int limi
     return best;
 }
 constexpr int ACC_LIMIT = 40;
+constexpr int ACC_LIMIT_HIGH = 64;
 
 template <int LA, int LB, int LC, int LD>
 struct ClassShape {
@@ -55,7 +56,11 @@ struct ClassShape {
     static constexpr int HCD = nherm(LCD);
     static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     static constexpr int NAB = NA * NB, NCD = NC * ND;
-    static constexpr int CH = chunk_size(NAB, HCD, NCD, ACC_LIMIT);   // bra components per item
+    // bra components per item: accumulators (ket Hermite + ket Cartesian
+    // per component) up to ACC_LIMIT doubles, or ACC_LIMIT_HIGH for L >= 4,
+    // whose R_tuv dominates the registers anyway and whose chunks each
+    // recompute Boys and R
+    static constexpr int CH = chunk_size(NAB, HCD, NCD, L >= 4 ? ACC_LIMIT_HIGH : ACC_LIMIT);
     static constexpr int NCHUNK = NAB / CH;
     static constexpr int NTB = LA + LB + 1, NTK = LC + LD + 1;
     static constexpr int EB = (LA + 1) * (LB + 1) * NTB;   // 1-D bra E table

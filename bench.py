@@ -38,11 +38,11 @@ FP64_PEAK_TFLOPS = 78.6
 # counters read for a copy of known size (recorded with the command lines in
 # profiles/); a class absent here reports null.
 TRAFFIC = {
-    (0, 0, 0, 0): 2.243e+07, (1, 0, 0, 0): 4.228e+07, (1, 0, 1, 0): 4.499e+07, (1, 1, 0, 0): 2.057e+07,
-    (1, 1, 1, 0): 5.509e+07, (1, 1, 1, 1): 7.417e+06, (2, 0, 0, 0): 1.914e+07, (2, 0, 1, 0): 3.835e+07,
-    (2, 0, 1, 1): 1.156e+07, (2, 0, 2, 0): 5.956e+06, (2, 1, 0, 0): 1.250e+07, (2, 1, 1, 0): 1.574e+07,
-    (2, 1, 1, 1): 7.839e+06, (2, 1, 2, 0): 5.723e+06, (2, 1, 2, 1): 6.062e+06, (2, 2, 0, 0): 3.400e+06,
-    (2, 2, 1, 0): 5.654e+06, (2, 2, 1, 1): 2.865e+06, (2, 2, 2, 0): 2.498e+06, (2, 2, 2, 1): 5.490e+06,
+    (0, 0, 0, 0): 2.242e+07, (1, 0, 0, 0): 4.229e+07, (1, 0, 1, 0): 4.499e+07, (1, 1, 0, 0): 2.063e+07,
+    (1, 1, 1, 0): 5.513e+07, (1, 1, 1, 1): 7.862e+06, (2, 0, 0, 0): 1.916e+07, (2, 0, 1, 0): 3.828e+07,
+    (2, 0, 1, 1): 1.154e+07, (2, 0, 2, 0): 5.946e+06, (2, 1, 0, 0): 1.251e+07, (2, 1, 1, 0): 1.652e+07,
+    (2, 1, 1, 1): 7.645e+06, (2, 1, 2, 0): 5.551e+06, (2, 1, 2, 1): 6.124e+06, (2, 2, 0, 0): 3.453e+06,
+    (2, 2, 1, 0): 5.270e+06, (2, 2, 1, 1): 2.754e+06, (2, 2, 2, 0): 2.329e+06, (2, 2, 2, 1): 5.559e+06,
 }
 
 

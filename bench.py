@@ -312,6 +312,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--serial-classes", action="store_true",
+                    help="run the Fock classes one after the other in the profiled step (for counter runs)")
     ap.add_argument("--workload", choices=("h2o8", "h2o16-tier", "batch512", "ao2mo", "copy-calibration"), default="h2o8",
                     help="h2o8: (H2O)8 cc-pVDZ FP64 (default); h2o16-tier: (H2O)16 cc-pVDZ FP64 vs FP32 tier; "
                          "batch512: 512 small 6-31G clusters")
@@ -351,7 +353,8 @@ def main():
     for s in range(args.steps):
         # per-class HIP events on the classes' own streams, inside the timed
         # region (last step)
-        mols.append(run_scf(geom, basis, profile=(not args.no_profile) and s == args.steps - 1))
+        mols.append(run_scf(geom, basis, profile=(not args.no_profile) and s == args.steps - 1,
+                            serial=args.serial_classes))
         print("step %d done" % s, file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
     scf_per_s = args.steps / elapsed

@@ -178,6 +178,16 @@ int mmd_fock_tiers(mmd_ctx *ctx, int64_t *nquart, int64_t *nquart_fp32);
 int mmd_set_streams(mmd_ctx *ctx, int n);
 
 /*
+ * LDS budget (bytes) of one Fock class launch; 0 restores the device's
+ * limit (64 KiB on gfx950).  A class whose bra rows and buffers do not fit
+ * runs without the ket-side Coulomb buffer, and without room for the rows it
+ * runs on the wavefront-per-quartet kernel with survivor lists; large bases
+ * take these paths by themselves, a smaller budget lets a small basis take
+ * them too (for testing them).  No counterpart in the reference.
+ */
+int mmd_set_lds_limit(mmd_ctx *ctx, int64_t bytes);
+
+/*
  * GPU-resident state of the direct RHF loop (mmd/scf.py:10-80 of the
  * reference; the caller runs the loop and the eigensolver).  No counterpart
  * in the reference: H, S, X = S^-1/2, F and the density stay in device

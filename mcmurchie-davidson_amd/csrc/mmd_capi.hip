@@ -152,7 +152,8 @@ struct mmd_ctx {
     int64_t unit_off[NQCLASS + 1] = {0};
     int unit_seg[NQCLASS] = {0};
     bool units_ok = false;
-    size_t max_lds = 65536;
+    size_t max_lds = 65536;             // LDS a class launch may use (mmd_set_lds_limit)
+    size_t max_lds_device = 65536;      // the device's limit per workgroup
     bool last_row_kernel[NQCLASS] = {};
     int64_t st_nquart[NQCLASS] = {0};
 
@@ -692,16 +693,22 @@ int mmd_ctx_create(mmd_ctx** out, int device_id)
     c->device = device_id;
     // Classes on more streams than the process has hardware queues would
     // share queues in an order the longest-first plan does not see: spread
-    // them over min(8, GPU_MAX_HW_QUEUES) streams (HIP's default is 4).
+    // them over min(8, GPU_MAX_HW_QUEUES) streams (HIP's default is 4).  The
+    // variable is read once, at the first context of the process, so that a
+    // later change of the environment (after HIP read it) does not matter.
     {
-        const char* hq = getenv("GPU_MAX_HW_QUEUES");
-        const int q = hq ? atoi(hq) : 4;
-        c->nstreams = std::max(1, std::min(q > 0 ? q : 4, 1 + mmd_ctx::NSIDE));
+        static const int hw_queues = [] {
+            const char* hq = getenv("GPU_MAX_HW_QUEUES");
+            const int q = hq ? atoi(hq) : 4;
+            return q > 0 ? q : 4;
+        }();
+        c->nstreams = std::max(1, std::min(hw_queues, 1 + mmd_ctx::NSIDE));
     }
     int lds_max = 0;
     if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device_id) == hipSuccess &&
         lds_max > 0)
         c->max_lds = std::min<size_t>((size_t)lds_max, 65536);
+    c->max_lds_device = c->max_lds;
     hipError_t e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
@@ -1134,6 +1141,14 @@ int mmd_set_streams(mmd_ctx* ctx, int n)
     if (!ctx) return fail(MMD_ERR_ARG, "null context");
     if (n < 1 || n > 1 + mmd_ctx::NSIDE) return fail(MMD_ERR_ARG, "streams must be in [1, %d]", 1 + mmd_ctx::NSIDE);
     ctx->nstreams = n;
+    return MMD_OK;
+}
+
+int mmd_set_lds_limit(mmd_ctx* ctx, int64_t bytes)
+{
+    if (!ctx) return fail(MMD_ERR_ARG, "null context");
+    if (bytes < 0) return fail(MMD_ERR_ARG, "LDS limit must be >= 0");
+    ctx->max_lds = bytes == 0 ? ctx->max_lds_device : std::min<size_t>((size_t)bytes, ctx->max_lds_device);
     return MMD_OK;
 }
 

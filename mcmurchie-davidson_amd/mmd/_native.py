@@ -27,11 +27,10 @@ if not os.path.exists(LIB_PATH):
         "libmmd.so not found at %s; build it with `make -C mcmurchie-davidson_amd/csrc` "
         "(or __graft_entry__.build())" % LIB_PATH)
 
-# The Fock build runs its shell-quartet classes side by side on eight
-# streams; HIP maps a process's streams onto GPU_MAX_HW_QUEUES hardware
-# queues (four unless set), so ask for eight unless the process chose a
-# value (it takes effect if HIP has not been initialised yet).
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+# The Fock build spreads its shell-quartet classes over min(8,
+# GPU_MAX_HW_QUEUES) streams (four when the variable is unset, HIP's own
+# default); a process that wants eight sets the variable before HIP starts.
+# The library itself leaves the environment alone.
 lib = ctypes.CDLL(LIB_PATH)
 
 _c_int = ctypes.c_int
@@ -57,6 +56,8 @@ lib.mmd_one_electron.restype = _c_int
 lib.mmd_one_electron.argtypes = [_ctx_p, _vp, _vp, _vp, _vp, _vp, _vp]
 lib.mmd_schwarz.restype = _c_int
 lib.mmd_schwarz.argtypes = [_ctx_p, _vp]
+lib.mmd_set_lds_limit.restype = _c_int
+lib.mmd_set_lds_limit.argtypes = [_ctx_p, _c_i64]
 lib.mmd_set_schwarz.restype = _c_int
 lib.mmd_set_schwarz.argtypes = [_ctx_p, _vp]
 lib.mmd_eri_tensor.restype = _c_int
@@ -113,7 +114,7 @@ EXPORTED = [
     "mmd_set_basis", "mmd_set_atoms", "mmd_one_electron", "mmd_schwarz", "mmd_set_schwarz",
     "mmd_eri_tensor", "mmd_eri_list", "mmd_fock_delta", "mmd_jk_tensor", "mmd_boys",
     "mmd_last_fock_ms", "mmd_set_profiling", "mmd_fock_stats", "mmd_set_precision", "mmd_fock_tiers",
-    "mmd_set_streams", "mmd_scf_begin", "mmd_scf_step", "mmd_scf_density", "mmd_scf_get", "mmd_ao2mo",
+    "mmd_set_streams", "mmd_set_lds_limit", "mmd_scf_begin", "mmd_scf_step", "mmd_scf_density", "mmd_scf_get", "mmd_ao2mo",
     "mmd_mlp_create", "mmd_mlp_destroy", "mmd_mlp_nparam", "mmd_mlp_set_params", "mmd_mlp_get_params",
     "mmd_mlp_forward", "mmd_mlp_loss_grad", "mmd_mlp_train",
 ]
@@ -338,6 +339,10 @@ class Context:
         self.last_ao2mo_ms = ms.value
         return out
 
+    def set_lds_limit(self, nbytes=0):
+        """LDS budget of a Fock class launch (0: the device's limit)."""
+        check(lib.mmd_set_lds_limit(self._h, int(nbytes)))
+
     def fock_tiers(self):
         """(surviving shell quartets, of which FP32) of the last symmetrised build."""
         a, b = _c_i64(0), _c_i64(0)
@@ -427,6 +432,70 @@ def default_context(device=0):
             ctx = Context(device)
             _default[device] = ctx
         return ctx
+
+
+class PerCallBasis(object):
+    """Device-resident basis for the per-call entry points (ERI(), S(), ...).
+
+    The Basis objects seen so far stay on the device of one context, so that
+    a loop over integrals of the same functions (the reference's
+    tests/test013.py style) uploads each function once instead of on every
+    call.  A function is recognised by identity and by its contents (origin,
+    exponents, weights), so an object changed in place is uploaded again.
+    """
+
+    MAX_FUNCTIONS = 512
+
+    def __init__(self, device=0):
+        self.ctx = Context(device)
+        self.bfs = []       # the resident functions, in device order (kept alive)
+        self.index = {}     # id -> (position, fingerprint)
+        self.lock = threading.Lock()
+
+    @staticmethod
+    def _fingerprint(b):
+        return (np.asarray(b.origin, dtype=np.float64).tobytes(), np.asarray(b.shell).tobytes(),
+                np.asarray(b.exps, dtype=np.float64).tobytes(),
+                (np.asarray(b.norm, dtype=np.float64) * np.asarray(b.coefs, dtype=np.float64)).tobytes())
+
+    def indices(self, fns):
+        """Positions of `fns` in the resident basis, uploading it again when
+        a function is new or changed.  Call with self.lock held."""
+        pos = []
+        stale = False
+        fps = [self._fingerprint(b) for b in fns]
+        for b, fp in zip(fns, fps):
+            hit = self.index.get(id(b))
+            if hit is None or hit[1] != fp or self.bfs[hit[0]] is not b:
+                stale = True
+                break
+            pos.append(hit[0])
+        if not stale:
+            return pos
+        if len(self.bfs) + len(fns) > self.MAX_FUNCTIONS:
+            self.bfs, self.index = [], {}
+        for b, fp in zip(fns, fps):
+            hit = self.index.get(id(b))
+            if hit is not None and self.bfs[hit[0]] is b:
+                if hit[1] != fp:       # changed in place: new slot
+                    self.bfs.append(b)
+                    self.index[id(b)] = (len(self.bfs) - 1, fp)
+            else:
+                self.bfs.append(b)
+                self.index[id(b)] = (len(self.bfs) - 1, fp)
+        self.ctx.set_basis_from_bfs(self.bfs)
+        return [self.index[id(b)][0] for b in fns]
+
+
+_percall = {}
+
+
+def per_call_basis(device=0):
+    with _lock:
+        pc = _percall.get(device)
+        if pc is None:
+            pc = _percall[device] = PerCallBasis(device)
+        return pc
 
 
 # Contexts keyed by the identity of a basis-function list, so that formPT,

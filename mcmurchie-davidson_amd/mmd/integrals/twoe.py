@@ -78,9 +78,10 @@ def ERI(a, b, c, d):
     for x in (a, b, c, d):
         if not isinstance(x, Basis):
             raise TypeError("Argument has incorrect type (expected Basis, got %s)" % type(x).__name__)
-    ctx = _native.default_context()
-    ctx.set_basis_from_bfs([a, b, c, d])
-    return float(ctx.eri_list(np.array([[0, 1, 2, 3]], dtype=np.int64))[0])
+    pc = _native.per_call_basis()
+    with pc.lock:
+        idx = pc.indices([a, b, c, d])
+        return float(pc.ctx.eri_list(np.array([idx], dtype=np.int64))[0])
 
 
 def doERIs(N, TwoE, bfs):

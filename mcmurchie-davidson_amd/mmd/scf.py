@@ -47,12 +47,14 @@ _EVD_LWORK = {}
 def _eigh(A):
     """Eigenvalues (ascending) and eigenvectors of a symmetric matrix by
     LAPACK's divide and conquer, as scipy.linalg.eigh(A, driver="evd")
-    computes them (same routine, lower triangle, same results), without its
-    argument checks and workspace query on every call; complex input goes to
-    scipy.linalg.eigh."""
+    computes them (same routine, lower triangle, same results, and the same
+    ValueError for non-finite input), without the workspace query on every
+    call; complex input goes to scipy.linalg.eigh."""
     A = np.asarray(A)
     if A.dtype != np.float64 or A.ndim != 2:
         return scipy.linalg.eigh(A, driver="evd")
+    if not np.isfinite(A).all():
+        raise ValueError("array must not contain infs or NaNs")
     n = A.shape[0]
     lw = _EVD_LWORK.get(n)
     if lw is None:

@@ -69,6 +69,8 @@ def lib():
         L.orc_form_pt.restype = _i64
         L.orc_form_pt.argtypes = bfs + [_i64, _pd, ctypes.c_void_p, _pd, _d, _pd,
                                         ctypes.c_void_p, _i64, _i64]
+        L.orc_fock_elements.restype = None
+        L.orc_fock_elements.argtypes = bfs + [_i64, _pd, _pd, _d, _i64, _pi, _pd]
         L.orc_one_electron.restype = None
         L.orc_one_electron.argtypes = bfs + [_i64, _i64, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd]
         _lib = L
@@ -257,6 +259,28 @@ class OracleMolecule:
         if cplx:
             G = G + 1j * Gi
         return G, nsurv
+
+
+def fock_elements(mol, P, S_diag, tol, pq):
+    """Elements (p, q) of the symmetrised formPT(P, 0) for a real symmetric
+    P (cython/fock.pyx:38-85, mmd/scf.py:93); S_diag is the full N*N
+    Schwarz diagonal (pq|pq)."""
+    N = mol.nbasis
+    pq = np.ascontiguousarray(np.asarray(pq, dtype=np.int64).reshape(-1, 2))
+    out = np.zeros(len(pq))
+    lib().orc_fock_elements(*mol._bfs(), N, np.ascontiguousarray(P, dtype=np.float64),
+                            np.ascontiguousarray(S_diag, dtype=np.float64), float(tol), len(pq), pq, out)
+    return out
+
+
+def schwarz_full(mol):
+    """The Schwarz diagonal as a full symmetric N*N matrix."""
+    N = mol.nbasis
+    s = mol.schwarz()
+    S = np.zeros((N, N))
+    iu = np.tril_indices(N)
+    S[iu] = s
+    return S + np.tril(S, -1).T
 
 
 def rhf(mol, DIIS=True, direct=False, conver=1e-8, acc2e=1e-12, maxiter=100, eri=None):

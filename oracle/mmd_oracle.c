@@ -22,6 +22,8 @@
  *                     integral test.
  *   orc_form_pt       sym(formPT(P,0)) == 2J-K from the full tensor, linearity,
  *                     and the reference's direct-SCF methane energy.
+ *   orc_fock_elements equal to the symmetrised orc_form_pt, element by element,
+ *                     with screening active.
  *   one-electron      S/T literals, energies.
  */
 #include <math.h>
@@ -340,6 +342,56 @@ int64_t orc_form_pt(ORC_BFS_ARGS, int64_t N, const double *dPr, const double *dP
 #undef CABS
 #undef M
     return survivors;
+}
+
+
+/* Selected elements of the symmetrised direct Fock contribution
+ * 0.5 (G + G^T) with G = formPT(P, 0) (cython/fock.pyx:38-85 followed by
+ * mmd/scf.py:93) for a real symmetric P, one element at a time:
+ *   G_pq = sum_rs [ 2 m(pq,rs) (pq|rs) - m(pr,qs) (pr|qs) ] P_rs
+ * where m(ij,kl) = 1 when formPT keeps the canonical quartet of (ij|kl):
+ *   sqrt(S_ij) sqrt(S_kl) max(4|P_ij|, 4|P_kl|, |P_ik|, |P_il|, |P_jk|, |P_jl|) >= tol
+ * (S = full N*N Schwarz diagonal).  The test is invariant under the eight
+ * index permutations for symmetric P, so every member of an orbit carries its
+ * canonical quartet's decision; formPT's degeneracy factors and -1/4
+ * exchange weights then sum to the expression above.  Costs about 1.5 N^2
+ * contracted ERIs per element, so a few elements of a large basis can be
+ * checked without the N^4 loop.  pq: 2*n indices; out: n values. */
+static int keep_quartet(const double *S, const double *P, int64_t N, double tol, int64_t i, int64_t j,
+                        int64_t k, int64_t l)
+{
+#define M(A, a, b) ((A)[(a) * N + (b)])
+    double bound = sqrt(M(S, i, j)) * sqrt(M(S, k, l));
+    double dm = 4.0 * fabs(M(P, i, j)), x;
+    x = 4.0 * fabs(M(P, k, l)); if (x > dm) dm = x;
+    x = fabs(M(P, i, k)); if (x > dm) dm = x;
+    x = fabs(M(P, i, l)); if (x > dm) dm = x;
+    x = fabs(M(P, j, k)); if (x > dm) dm = x;
+    x = fabs(M(P, j, l)); if (x > dm) dm = x;
+#undef M
+    return bound * dm >= tol;
+}
+
+void orc_fock_elements(ORC_BFS_ARGS, int64_t N, const double *P, const double *S, double tol, int64_t n,
+                       const int64_t *pq, double *out)
+{
+    ORC_BFS_INIT;
+    for (int64_t e = 0; e < n; ++e) {
+        const int64_t p = pq[2 * e], q = pq[2 * e + 1];
+        double g = 0.0;
+        for (int64_t r = 0; r < N; ++r)
+            for (int64_t s = 0; s <= r; ++s) {
+                if (!keep_quartet(S, P, N, tol, p, q, r, s)) continue;
+                const double w = (r == s ? 1.0 : 2.0) * P[r * N + s];
+                if (w != 0.0) g += 2.0 * w * eri_bf(&B, p, q, r, s);
+            }
+        for (int64_t r = 0; r < N; ++r)
+            for (int64_t s = 0; s < N; ++s) {
+                if (P[r * N + s] == 0.0 || !keep_quartet(S, P, N, tol, p, r, q, s)) continue;
+                g -= P[r * N + s] * eri_bf(&B, p, r, q, s);
+            }
+        out[e] = g;
+    }
 }
 
 /* ------------------------------------------------------------------ */

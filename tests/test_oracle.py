@@ -137,6 +137,57 @@ def test_nuclear_attraction_derivative_relation(oracle):
         assert (4 * d2 - d1) / 3 / (2 * a) == pytest.approx(prim(l), abs=1e-12)
 
 
+def test_angular_momentum_from_overlap_derivatives(oracle):
+    """RxDel (cython/onee.pyx:36-43, 139-192) for single primitives of every
+    (l, l') <= d against the overlap:
+      L_x(a, b) = -<a| (y - C_y) d/dz - (z - C_z) d/dy |b>,
+      <a| (y - C_y) d/dz |b> = -d/dB_z [ <a y+1| b> + (A_y - C_y) <a|b> ]
+    for unnormalised primitives (d/dz b(r - B) = -d/dB_z b), with the centre
+    derivative taken by Richardson-extrapolated central differences of the
+    oracle's overlap, and L antisymmetric."""
+    A0 = np.array([0.3, -0.2, 0.5])
+    B0 = np.array([-0.4, 0.6, 0.1])
+    C = np.array([0.25, -0.15, 0.35])
+    a, b = 0.9, 1.4
+    shells = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (2, 0, 0), (1, 1, 0), (0, 1, 2)]
+
+    def ints(la, lb, B):
+        m = _Mini(oracle, [(A0, la, [a], [1.0]), (B, lb, [b], [1.0])]).o
+        m.center_of_charge = C
+        S, T, V, M, L = m.one_electron()
+        w = m.norm * m.coefs
+        return S[0, 1] / (w[0] * w[1]), L[:, 0, 1] / (w[0] * w[1])
+
+    def s_prim(la, lb, B):
+        return ints(la, lb, B)[0]
+
+    def d_dB(la, lb, x):
+        def f(h):
+            B = B0.copy()
+            B[x] += h
+            return s_prim(la, lb, B)
+        d1 = (f(2e-3) - f(-2e-3)) / 4e-3
+        d2 = (f(1e-3) - f(-1e-3)) / 2e-3
+        return (4 * d2 - d1) / 3
+
+    def r_grad(la, lb, y, z):
+        # <a| (r_y - C_y) d/dr_z |b>
+        up = list(la)
+        up[y] += 1
+        return -(d_dB(tuple(up), lb, z) + (A0[y] - C[y]) * d_dB(la, lb, z))
+
+    for la in shells:
+        for lb in shells:
+            _, L = ints(la, lb, B0)
+            for x, (y, z) in enumerate(((1, 2), (2, 0), (0, 1))):
+                expect = -(r_grad(la, lb, y, z) - r_grad(la, lb, z, y))
+                assert L[x] == pytest.approx(expect, abs=1e-9)
+    # antisymmetry on a contracted cluster basis
+    om = oracle.OracleMolecule(WATER, _gbs("cc-pvdz"))
+    L = om.one_electron()[4]
+    assert np.max(np.abs(L + L.transpose(0, 2, 1))) < 1e-13
+
+
 # --------------------------------------------------------------------------
 # ERIs
 # --------------------------------------------------------------------------
@@ -272,6 +323,26 @@ def test_formpt_equals_2j_minus_k(oracle):
     G2, _ = om.form_pt(Q.astype(complex), np.zeros_like(P, complex), om.schwarz(), 0.0)
     G3, _ = om.form_pt((P + Q).astype(complex), np.zeros_like(P, complex), om.schwarz(), 0.0)
     assert np.max(np.abs(G1 + G2 - G3)) < 1e-13
+
+
+def test_fock_elements_equal_formpt(oracle):
+    """The elementwise restatement of the symmetrised formPT (used for the
+    large-basis fixtures) equals the symmetrised orc_form_pt with the screen
+    active (tol 1e-12, and 1e-6 where it drops quartets)."""
+    om = oracle.OracleMolecule(WATER, _gbs("cc-pvdz"))
+    N = om.nbasis
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((N, N))
+    P = (A + A.T) * 1e-3
+    S = oracle.schwarz_full(om)
+    pq = [(p, q) for p in range(0, N, 3) for q in range(0, p + 1, 2)]
+    for tol in (1e-12, 1e-6):
+        G, nsurv = om.form_pt(P.astype(complex), np.zeros((N, N), complex), om.schwarz(), tol)
+        Gs = 0.5 * (G + G.T).real
+        e = oracle.fock_elements(om, P, S, tol, pq)
+        ref = np.array([Gs[p, q] for p, q in pq])
+        assert np.max(np.abs(e - ref)) < 1e-15
+    assert nsurv < N * (N + 1) // 2 * (N * (N + 1) // 2 + 1) // 2
 
 
 def test_rhf_water_sto3g_reference(oracle):

@@ -119,6 +119,23 @@ int mmd_fock_delta(mmd_ctx *ctx, const double *dP_re, const double *dP_im, doubl
                    double *G_im, int on_device, int64_t *nsurv);
 
 /*
+ * mmd_fock_delta's symmetrised build (raw = 0, real dP) that allocates
+ * nothing: no counterpart in the reference; it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  dP and G are N*N device pointers; nsurv may be
+ * NULL.  The kernels are mmd_fock_delta's; their work buffers (|dP|, shell
+ * maxima, the unsymmetrised accumulator, counters, survivor lists) live in a
+ * static 16 MiB array per device instead of the context, so the basis is
+ * limited to what fits there (about 700 functions with the default LDS
+ * budget; a larger basis, or survivor lists beyond the array, returns
+ * MMD_ERR_ARG).  The context must have its Schwarz screen (mmd_schwarz or
+ * mmd_set_schwarz).  Every call on one GPU shares that GPU's array: calls
+ * that overlap in time on one GPU (from several threads or contexts) race
+ * and must be serialised by the caller.  Synchronous, like the other calls.
+ */
+int mmd_fock_delta_noalloc(mmd_ctx *ctx, const double *dP, double tol, double *G, int64_t *nsurv);
+
+/*
  * Conventional J and K from a stored tensor (mmd/scf.py:97-98):
  *   J_pq = sum_rs (pq|rs) P_sr,   K_pq = sum_rs (ps|qr) P_sr
  * T: N^4 (host or device per t_on_device); P, J, K: N*N host pointers;

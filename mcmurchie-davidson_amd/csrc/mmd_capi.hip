@@ -11,6 +11,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -76,8 +77,43 @@ struct DevBuf {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Static scratch of mmd_fock_delta_noalloc: one array per device (each GPU
+// the library runs on holds its own copy until the process ends).  Host code
+// never passes the array's name to a kernel (on the host it is a stand-in,
+// not a device address): every use goes through the address that
+// hipGetSymbolAddress returned for the device, looked up once per device when
+// the first context on it is created (scratch_for_device).
+// ---------------------------------------------------------------------------
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(16) << 20;   // 16 MiB
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device (device_id), looked up
+// on the first call for that device.
+int scratch_for_device(int device_id, void** out)
+{
+    if (device_id < 0 || device_id >= MAX_DEVICES) return fail(MMD_ERR_ARG, "device id out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device_id]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(MMD_ERR_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_ptr[device_id] = ptr;
+    }
+    *out = g_scratch_ptr[device_id];
+    return MMD_OK;
+}
+}  // namespace
+
 struct mmd_ctx {
     int device = 0;
+    void* noalloc_scratch = nullptr;    // this device's `scratch` (device address)
     hipStream_t st = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // side streams for concurrent class kernels (default priority), with the
@@ -363,7 +399,6 @@ int build_pairs(mmd_ctx* c, const std::vector<double>* qpair)
         c->qc_total[qc] = (pcb == pck) ? nb * (nb + 1) / 2 : nb * nk;
         c->qc_off[qc + 1] = c->qc_off[qc] + c->qc_total[qc];
     }
-    HIP_TRY(c->d_surv.reserve((size_t)std::max<int64_t>(c->qc_off[NQCLASS], (int64_t)c->pairs.size())));
     HIP_TRY(c->d_count.reserve(NQCLASS));
     c->pairs_ok = true;
     c->units_ok = false;
@@ -457,18 +492,30 @@ ClassArgs base_args(mmd_ctx* c)
     return A;
 }
 
+// Device memory a Fock build works in besides its inputs and outputs: work-unit
+// counters, survivor counts and statistics, the shell-block maxima of |dP|,
+// and the survivor lists of the classes that run on the wavefront-per-quartet
+// kernel (class qc at surv + surv_off[qc]).  The context's buffers, or a
+// caller-free static array (mmd_fock_delta_noalloc).
+struct FockScratch {
+    int* rowctr = nullptr;                 // 2 * NQCLASS
+    int* count = nullptr;                  // NQCLASS
+    unsigned long long* nprim = nullptr;   // 3 * NQCLASS
+    double* Dsh = nullptr;                 // nshell^2
+    int2* surv = nullptr;
+    int64_t surv_off[NQCLASS] = {0};       // survivor lists, packed: wave classes only
+    int64_t surv_cap = -1;                 // entries available at surv; < 0: grow the context's buffer
+    bool allow_measure = true;             // may create events and re-measure the stream plan
+};
+
 // Launch the Fock kernels of every class.  Classes with a row kernel screen
 // inline; the wavefront-per-quartet classes (and any class whose bra rows do
 // not fit in LDS) are screened into survivor lists first.
-int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol)
+int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol, FockScratch& W0)
 {
-    HIP_TRY(c->d_rowctr.reserve(2 * NQCLASS));
-    HIP_TRY(c->d_nprim.reserve(3 * NQCLASS));
-    HIP_TRY(launch_fill_int(2 * NQCLASS, c->d_rowctr.p, 0, c->st));
-    HIP_TRY(launch_fill_int(NQCLASS, c->d_count.p, 0, c->st));
-    HIP_TRY(launch_fill_int(6 * NQCLASS, reinterpret_cast<int*>(c->d_nprim.p), 0, c->st));
     const bool tier32 = c->fp32_below > tol;
     if (!c->units_ok) {
+        if (!W0.allow_measure) return fail(MMD_ERR_STATE, "work units are built by mmd_schwarz / mmd_set_schwarz");
         const int rc = build_units(c);
         if (rc) return rc;
     }
@@ -486,6 +533,24 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol)
         row_kernel[qc] = !fock_class_is_wave(la, lb, lc, ld) &&
                          (kj_lds[qc] || fock_class_lds(qc, 0, c->nbf, maxpp[qc], 0) <= c->max_lds);
     }
+    {
+        // survivor lists only for the classes that use them, packed
+        int64_t off = 0;
+        for (int qc = 0; qc < NQCLASS; ++qc) {
+            W0.surv_off[qc] = off;
+            if (c->qc_total[qc] && !row_kernel[qc]) off += c->qc_total[qc];
+        }
+        if (W0.surv_cap < 0) {
+            HIP_TRY(c->d_surv.reserve((size_t)std::max<int64_t>(off, 1)));
+            W0.surv = c->d_surv.p;
+        } else if (off > W0.surv_cap) {
+            return fail(MMD_ERR_ARG, "survivor lists need %lld entries, the scratch holds %lld", (long long)off,
+                        (long long)W0.surv_cap);
+        }
+    }
+    HIP_TRY(launch_fill_int(2 * NQCLASS, W0.rowctr, 0, c->st));
+    HIP_TRY(launch_fill_int(NQCLASS, W0.count, 0, c->st));
+    HIP_TRY(launch_fill_int(6 * NQCLASS, reinterpret_cast<int*>(W0.nprim), 0, c->st));
     // Classes run concurrently on the main and side streams (longest first,
     // each to the least loaded stream), so that the small classes fill the
     // GPU beside the large ones.  Profiling mode 1 brackets every class with
@@ -512,7 +577,7 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol)
     // misses the latency floor of the small classes (150-300 us each at
     // (H2O)8/cc-pVDZ, whatever their work), which left the longest stream
     // twice as long as the others.  One build in PLAN_EVERY measures again.
-    const bool measure = c->profiling == 0 && nstream > 1 && c->plan_countdown <= 0;
+    const bool measure = W0.allow_measure && c->profiling == 0 && nstream > 1 && c->plan_countdown <= 0;
     bool measured = nord > 0;
     for (int oi = 0; oi < nord; ++oi) measured = measured && c->plan_ms[order[oi]] > 0.0;
     if (measured)
@@ -542,7 +607,8 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol)
         qclass_decode(qc, &la, &lb, &lc, &ld);
         const int nb = c->pc_off[pcb + 1] - c->pc_off[pcb];
         const int nk = c->pc_off[pck + 1] - c->pc_off[pck];
-        if (c->profiling || measure) HIP_TRY(hipEventRecord(c->evc[2 * qc], sq));
+        const bool events = measure || (c->profiling && W0.allow_measure);
+        if (events) HIP_TRY(hipEventRecord(c->evc[2 * qc], sq));
         if (row_kernel[qc]) {
             RowArgs R;
             R.boff = c->pc_off[pcb];
@@ -551,8 +617,8 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol)
             R.nk = nk;
             R.tri = pcb == pck ? 1 : 0;
             R.nshell = (int)c->shells.size();
-            R.Dsh = c->d_Dsh.p;
-            R.row_counter = c->d_rowctr.p + qc;
+            R.Dsh = W0.Dsh;
+            R.row_counter = W0.rowctr + qc;
             R.maxpp = maxpp[qc];
             R.kj = kj_lds[qc] ? 1 : 0;
             R.seg = c->unit_seg[qc];
@@ -560,11 +626,11 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol)
             const int64_t units = c->unit_off[qc + 1] - c->unit_off[qc];
             R.nunits = (int)units;
             if (units == 0) {
-                if (c->profiling || measure) HIP_TRY(hipEventRecord(c->evc[2 * qc + 1], sq));
+                if (events) HIP_TRY(hipEventRecord(c->evc[2 * qc + 1], sq));
                 continue;
             }
-            R.stat_nquart = c->d_nprim.p + NQCLASS + qc;
-            R.stat_nprim = c->profiling ? c->d_nprim.p + qc : nullptr;
+            R.stat_nquart = W0.nprim + NQCLASS + qc;
+            R.stat_nprim = c->profiling ? W0.nprim + qc : nullptr;
             R.tier_lo = tier32 ? c->fp32_below : 0.0;
             R.tier_hi = HUGE_VAL;
             const int grid = (int)std::min<int64_t>(units, 2048);
@@ -573,22 +639,22 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol)
                 // FP32 tier: the same units, quartets with tol <= bound < fp32_below
                 R.tier_lo = 0.0;
                 R.tier_hi = c->fp32_below;
-                R.row_counter = c->d_rowctr.p + NQCLASS + qc;
-                R.stat_nquart = c->d_nprim.p + 2 * NQCLASS + qc;
+                R.row_counter = W0.rowctr + NQCLASS + qc;
+                R.stat_nquart = W0.nprim + 2 * NQCLASS + qc;
                 HIP_TRY(launch_fock_class(qc, 1, A, R, grid, sq));
             }
         } else {
+            int2* surv = W0.surv + W0.surv_off[qc];
             HIP_TRY(launch_screen_class(c->pc_off[pcb], nb, c->pc_off[pck], nk, pcb == pck ? 1 : 0, c->d_pairs.p,
-                                        c->d_Dsh.p, (int)c->shells.size(), tol, 0, c->d_surv.p + c->qc_off[qc],
-                                        c->d_count.p + qc, sq));
+                                        W0.Dsh, (int)c->shells.size(), tol, 0, surv, W0.count + qc, sq));
             ClassArgs W = A;
-            W.quart = c->d_surv.p + c->qc_off[qc];
-            W.count = c->d_count.p + qc;
+            W.quart = surv;
+            W.count = W0.count + qc;
             if (c->profiling)
-                HIP_TRY(launch_count_prims(W.quart, W.count, c->d_pairs.p, c->d_nprim.p + qc, sq));
+                HIP_TRY(launch_count_prims(W.quart, W.count, c->d_pairs.p, W0.nprim + qc, sq));
             HIP_TRY(launch_fock_wave_class(la, lb, lc, ld, W, c->qc_total[qc], sq));
         }
-        if (c->profiling || measure) HIP_TRY(hipEventRecord(c->evc[2 * qc + 1], sq));
+        if (events) HIP_TRY(hipEventRecord(c->evc[2 * qc + 1], sq));
     }
     for (int i = 1; i < nstream; ++i) {
         if (!used[i]) continue;
@@ -652,7 +718,11 @@ int set_schwarz(mmd_ctx* ctx, const std::vector<double>& h)
                 }
             q[pidx] = m;
         }
-    return build_pairs(ctx, &q);
+    const int rc = build_pairs(ctx, &q);
+    if (rc) return rc;
+    // the Fock build's work units belong to the pair table: built here, at
+    // set-up, so that a build (mmd_fock_delta_noalloc) allocates nothing
+    return build_units(ctx);
 }
 
 }  // namespace
@@ -727,6 +797,12 @@ int mmd_ctx_create(mmd_ctx** out, int device_id)
     if (e != hipSuccess) {
         mmd_ctx_destroy(c);
         return fail(MMD_ERR_DEVICE, "context setup: %s", hipGetErrorString(e));
+    }
+    // this device's address of the static scratch (once per device)
+    const int rc = scratch_for_device(device_id, &c->noalloc_scratch);
+    if (rc) {
+        mmd_ctx_destroy(c);
+        return rc;
     }
     *out = c;
     return MMD_OK;
@@ -1002,7 +1078,14 @@ int mmd_fock_delta(mmd_ctx* ctx, const double* dP_re, const double* dP_im, doubl
         A.D = ctx->d_D.p;
         A.G = ctx->d_G.p;
         HIP_TRY(launch_fill(nn, ctx->d_G.p, 0.0, ctx->st));
-        rc = run_fock_classes(ctx, A, tol);
+        HIP_TRY(ctx->d_rowctr.reserve(2 * NQCLASS));
+        HIP_TRY(ctx->d_nprim.reserve(3 * NQCLASS));
+        FockScratch W;
+        W.rowctr = ctx->d_rowctr.p;
+        W.count = ctx->d_count.p;
+        W.nprim = ctx->d_nprim.p;
+        W.Dsh = ctx->d_Dsh.p;
+        rc = run_fock_classes(ctx, A, tol, W);
         if (rc) return rc;
         HIP_TRY(launch_symmetrize(N, ctx->d_G.p, ctx->d_Gout.p, ctx->st));
         HIP_TRY(hipMemcpyAsync(G_re, ctx->d_Gout.p, sizeof(double) * nn, k_out, ctx->st));
@@ -1050,6 +1133,82 @@ int mmd_fock_delta(mmd_ctx* ctx, const double* dP_re, const double* dP_im, doubl
         --ctx->plan_countdown;
     }
     if (ctx->profiling && !raw) return collect_profile(ctx);
+    return MMD_OK;
+}
+
+// The symmetrised build of mmd_fock_delta (raw = 0, real dP, device
+// pointers) with every work buffer in the static `scratch` of this device
+// instead of the context's: |dP|, the shell-block maxima, the unsymmetrised
+// accumulator, counters and survivor lists.  No allocation on the host or the
+// device; the kernels are the same.
+int mmd_fock_delta_noalloc(mmd_ctx* ctx, const double* dP, double tol, double* G, int64_t* nsurv)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!ctx->schwarz_ok) return fail(MMD_ERR_STATE, "mmd_schwarz must be called before mmd_fock_delta_noalloc");
+    if (!ctx->pairs_ok) return fail(MMD_ERR_UNSUPPORTED, "%s", ctx->shells_err.c_str());
+    if (!ctx->units_ok) return fail(MMD_ERR_STATE, "work units missing (set the Schwarz screen again)");
+    if (!dP || !G) return fail(MMD_ERR_ARG, "null pointer");
+    if (!ctx->noalloc_scratch) return fail(MMD_ERR_STATE, "no scratch address for this device");
+    const int N = ctx->nbf;
+    const size_t nn = (size_t)N * N;
+    const size_t ns = ctx->shells.size();
+    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+    unsigned char* base = static_cast<unsigned char*>(ctx->noalloc_scratch);
+    size_t off = 0;
+    double* absD = reinterpret_cast<double*>(base + off);
+    off += up(sizeof(double) * nn);
+    double* Gacc = reinterpret_cast<double*>(base + off);
+    off += up(sizeof(double) * nn);
+    double* Dsh = reinterpret_cast<double*>(base + off);
+    off += up(sizeof(double) * ns * ns);
+    int* rowctr = reinterpret_cast<int*>(base + off);
+    off += up(sizeof(int) * 2 * NQCLASS);
+    int* count = reinterpret_cast<int*>(base + off);
+    off += up(sizeof(int) * NQCLASS);
+    unsigned long long* nprim = reinterpret_cast<unsigned long long*>(base + off);
+    off += up(sizeof(unsigned long long) * 3 * NQCLASS);
+    if (off > NOALLOC_SCRATCH_BYTES)
+        return fail(MMD_ERR_ARG, "a basis of %d functions needs %zu bytes of scratch; the limit is %zu", N, off,
+                    NOALLOC_SCRATCH_BYTES);
+    FockScratch W;
+    W.rowctr = rowctr;
+    W.count = count;
+    W.nprim = nprim;
+    W.Dsh = Dsh;
+    W.surv = reinterpret_cast<int2*>(base + off);
+    W.surv_cap = (int64_t)((NOALLOC_SCRATCH_BYTES - off) / sizeof(int2));
+    W.allow_measure = false;
+    HIP_TRY(hipEventRecord(ctx->ev0, ctx->st));
+    HIP_TRY(launch_abs_density((int)nn, dP, nullptr, absD, ctx->st));
+    HIP_TRY(launch_shell_block_max((int)ns, ctx->d_sh_bf0.p, ctx->d_sh_nbf.p, N, absD, Dsh, ctx->st));
+    ClassArgs A = base_args(ctx);
+    A.absD = absD;
+    A.sqS = ctx->d_sqS.p;
+    A.tol = tol;
+    A.D = dP;
+    A.G = Gacc;
+    HIP_TRY(launch_fill(nn, Gacc, 0.0, ctx->st));
+    rc = run_fock_classes(ctx, A, tol, W);
+    if (rc) return rc;
+    HIP_TRY(launch_symmetrize(N, Gacc, G, ctx->st));
+    HIP_TRY(hipEventRecord(ctx->ev1, ctx->st));
+    int h[NQCLASS];
+    unsigned long long hq[2 * NQCLASS];
+    HIP_TRY(hipMemcpyAsync(h, count, sizeof(int) * NQCLASS, hipMemcpyDeviceToHost, ctx->st));
+    HIP_TRY(hipMemcpyAsync(hq, nprim + NQCLASS, sizeof(hq), hipMemcpyDeviceToHost, ctx->st));
+    HIP_TRY(hipStreamSynchronize(ctx->st));
+    int64_t sum = 0, s32 = 0;
+    for (int i = 0; i < NQCLASS; ++i) {
+        sum += (int64_t)h[i] + (int64_t)hq[i] + (int64_t)hq[NQCLASS + i];
+        s32 += (int64_t)hq[NQCLASS + i];
+    }
+    ctx->last_nsurv = sum;
+    ctx->last_nsurv_fp32 = s32;
+    if (nsurv) *nsurv = sum;
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_fock_ms = ms;
     return MMD_OK;
 }
 

@@ -112,7 +112,7 @@ lib.mmd_mlp_train.argtypes = [_vp, _c_int, _vp, _vp, _c_int, _c_int, ctypes.c_fl
 EXPORTED = [
     "mmd_last_error", "mmd_version", "mmd_device_count", "mmd_ctx_create", "mmd_ctx_destroy",
     "mmd_set_basis", "mmd_set_atoms", "mmd_one_electron", "mmd_schwarz", "mmd_set_schwarz",
-    "mmd_eri_tensor", "mmd_eri_list", "mmd_fock_delta", "mmd_jk_tensor", "mmd_boys",
+    "mmd_eri_tensor", "mmd_eri_list", "mmd_fock_delta", "mmd_fock_delta_noalloc", "mmd_jk_tensor", "mmd_boys",
     "mmd_last_fock_ms", "mmd_set_profiling", "mmd_fock_stats", "mmd_set_precision", "mmd_fock_tiers",
     "mmd_set_streams", "mmd_set_lds_limit", "mmd_scf_begin", "mmd_scf_step", "mmd_scf_density", "mmd_scf_get", "mmd_ao2mo",
     "mmd_mlp_create", "mmd_mlp_destroy", "mmd_mlp_nparam", "mmd_mlp_set_params", "mmd_mlp_get_params",

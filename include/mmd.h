@@ -231,11 +231,33 @@ int mmd_scf_get(mmd_ctx *ctx, double *F, double *P, double *P_old, double *G);
  * basis (PostSCF.ao2mo, mmd/postscf.py:21-28):
  *   out[P,Q,R,S] = sum_pqrs C[p,P] C[q,Q] C[r,R] C[s,S] T[p,q,r,s]
  * T, out: N^4 row-major (host or device per the flags); C: N*N host.  Four
- * quarter transforms, each one FP64 GEMM (rocBLAS) of N x N^3 by N x N;
+ * quarter transforms, each one FP64 matrix-core product (mmd_dgemm's
+ * routine) of N x N by N x N^3;
  * gemm_ms (may be NULL) receives their device time.
  */
 int mmd_ao2mo(mmd_ctx *ctx, const double *T, int t_on_device, const double *C, double *out, int out_on_device,
               double *gemm_ms);
+
+/*
+ * Overlap powers X = S^-1/2 and U = S^1/2 (Molecule.one_electron_integrals,
+ * X = fractional_matrix_power(S, -0.5), U = fractional_matrix_power(S, 0.5),
+ * mmd/molecule.py:284-286), computed on the GPU by Newton-Schulz iterations
+ * of FP64 matrix-core products.  S: N*N host matrix, or NULL for the overlap
+ * of the last mmd_one_electron (still on the device); X, U: N*N host
+ * outputs (either may be NULL); iterations (may be NULL) receives the number
+ * of iterations.  MMD_ERR_ARG if S is not positive definite.
+ */
+int mmd_overlap_powers(mmd_ctx *ctx, const double *S, double *X, double *U, int *iterations);
+
+/*
+ * Row-major FP64 product C = alpha op(A) op(B) + beta C on the matrix cores
+ * (v_mfma_f64_16x16x4_f64), op = transpose when trans* = 1: the routine of
+ * mmd_ao2mo and mmd_overlap_powers.  No counterpart in the reference (its
+ * products are numpy's).  on_device = 1: A, B, C are device pointers;
+ * 0: host pointers (copied for the call).
+ */
+int mmd_dgemm(mmd_ctx *ctx, int transA, int transB, int M, int N, int K, double alpha, const double *A, int lda,
+              const double *B, int ldb, double beta, double *C, int ldc, int on_device);
 
 /*
  * Multilayer perceptron of the learned initial density (no counterpart in

@@ -18,6 +18,7 @@
 #include "../../include/mmd.h"
 #include <rocblas/rocblas.h>
 #include "mmd_device.h"
+#include "mmd_gemm.h"
 #include "mmd_kernels.h"
 #include "mmd_scf.h"
 #include "mmd_types.h"
@@ -151,6 +152,7 @@ struct mmd_ctx {
     DevBuf<ShellPair> d_pairs;
     DevBuf<PrimPair> d_pp;
     DevBuf<double> d_onee;   // S, T, V, M (3), L (3) of mmd_one_electron
+    DevBuf<double> d_ns;     // overlap powers: S, X, U and the Newton-Schulz work (8 N^2 + 2)
     int64_t npp_total = 0;
 
     // Schwarz
@@ -1531,12 +1533,6 @@ int mmd_ao2mo(mmd_ctx* ctx, const double* T, int t_on_device, const double* C, d
     if (!T || !C || !out) return fail(MMD_ERR_ARG, "null pointer");
     const int N = ctx->nbf;
     const size_t nn = (size_t)N * N, n3 = nn * N, n4 = nn * nn;
-    if (!ctx->blas) {
-        if (rocblas_create_handle(&ctx->blas) != rocblas_status_success)
-            return fail(MMD_ERR_DEVICE, "rocblas_create_handle failed");
-    }
-    if (rocblas_set_stream(ctx->blas, ctx->st) != rocblas_status_success)
-        return fail(MMD_ERR_DEVICE, "rocblas_set_stream failed");
     if (n3 > (size_t)INT32_MAX) return fail(MMD_ERR_UNSUPPORTED, "basis too large for the transform");
     DevBuf<double> a, b, dC;
     HIP_TRY(a.reserve(n4));
@@ -1552,7 +1548,7 @@ int mmd_ao2mo(mmd_ctx* ctx, const double* T, int t_on_device, const double* C, d
     double* y = b.p;
     HIP_TRY(hipEventRecord(ctx->ev0, ctx->st));
     for (int pass = 0; pass < 4; ++pass) {
-        if ((rc = gemm_rm(ctx, true, true, N, (int)n3, N, dC.p, N, x, N, y, (int)n3))) return rc;
+        HIP_TRY(launch_gemm_f64(true, true, N, (int)n3, N, 1.0, dC.p, N, x, N, 0.0, y, (int)n3, ctx->st));
         std::swap(x, y);
     }
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->st));
@@ -1564,6 +1560,64 @@ int mmd_ao2mo(mmd_ctx* ctx, const double* T, int t_on_device, const double* C, d
         HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         *gemm_ms = ms;
     }
+    return MMD_OK;
+}
+
+int mmd_overlap_powers(mmd_ctx* ctx, const double* S, double* X, double* U, int* iterations)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (ctx->nbf == 0) return fail(MMD_ERR_STATE, "no basis set");
+    if (!X && !U) return fail(MMD_ERR_ARG, "null pointer");
+    const int N = ctx->nbf;
+    const size_t nn = (size_t)N * N;
+    if (!S && ctx->d_onee.n < nn) return fail(MMD_ERR_STATE, "no overlap matrix: call mmd_one_electron or pass S");
+    HIP_TRY(ctx->d_ns.reserve(8 * nn + 2));
+    double* dS = ctx->d_ns.p;
+    double* dX = dS + nn;
+    double* dU = dX + nn;
+    if (S)
+        HIP_TRY(hipMemcpyAsync(dS, S, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->st));
+    else   // the overlap of the last mmd_one_electron, still on the device
+        HIP_TRY(hipMemcpyAsync(dS, ctx->d_onee.p, sizeof(double) * nn, hipMemcpyDeviceToDevice, ctx->st));
+    const int it = overlap_powers_device(N, dS, dX, dU, dU + nn, 100, ctx->st);
+    if (it < 0) return fail(MMD_ERR_DEVICE, "overlap powers: kernel launch failed");
+    if (it == 0)
+        return fail(MMD_ERR_ARG, "overlap matrix is not positive definite (Newton-Schulz did not converge)");
+    if (X) HIP_TRY(hipMemcpyAsync(X, dX, sizeof(double) * nn, hipMemcpyDeviceToHost, ctx->st));
+    if (U) HIP_TRY(hipMemcpyAsync(U, dU, sizeof(double) * nn, hipMemcpyDeviceToHost, ctx->st));
+    HIP_TRY(hipStreamSynchronize(ctx->st));
+    if (iterations) *iterations = it;
+    return MMD_OK;
+}
+
+int mmd_dgemm(mmd_ctx* ctx, int transA, int transB, int M, int N, int K, double alpha, const double* A, int lda,
+              const double* B, int ldb, double beta, double* C, int ldc, int on_device)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (M < 0 || N < 0 || K < 0) return fail(MMD_ERR_ARG, "negative dimension");
+    if (!A || !B || !C) return fail(MMD_ERR_ARG, "null pointer");
+    const int rA = transA ? K : M, cA = transA ? M : K;
+    const int rB = transB ? N : K, cB = transB ? K : N;
+    if (lda < std::max(1, cA) || ldb < std::max(1, cB) || ldc < std::max(1, N))
+        return fail(MMD_ERR_ARG, "leading dimension too small");
+    if (on_device) {
+        HIP_TRY(launch_gemm_f64(transA, transB, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, ctx->st));
+        HIP_TRY(hipStreamSynchronize(ctx->st));
+        return MMD_OK;
+    }
+    const size_t nA = (size_t)rA * lda, nB = (size_t)rB * ldb, nC = (size_t)M * ldc;
+    DevBuf<double> a, b, c;
+    HIP_TRY(a.reserve(nA));
+    HIP_TRY(b.reserve(nB));
+    HIP_TRY(c.reserve(nC));
+    HIP_TRY(hipMemcpyAsync(a.p, A, sizeof(double) * nA, hipMemcpyHostToDevice, ctx->st));
+    HIP_TRY(hipMemcpyAsync(b.p, B, sizeof(double) * nB, hipMemcpyHostToDevice, ctx->st));
+    HIP_TRY(hipMemcpyAsync(c.p, C, sizeof(double) * nC, hipMemcpyHostToDevice, ctx->st));
+    HIP_TRY(launch_gemm_f64(transA, transB, M, N, K, alpha, a.p, lda, b.p, ldb, beta, c.p, ldc, ctx->st));
+    HIP_TRY(hipMemcpyAsync(C, c.p, sizeof(double) * nC, hipMemcpyDeviceToHost, ctx->st));
+    HIP_TRY(hipStreamSynchronize(ctx->st));
     return MMD_OK;
 }
 

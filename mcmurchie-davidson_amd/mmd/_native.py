@@ -56,6 +56,11 @@ lib.mmd_one_electron.restype = _c_int
 lib.mmd_one_electron.argtypes = [_ctx_p, _vp, _vp, _vp, _vp, _vp, _vp]
 lib.mmd_schwarz.restype = _c_int
 lib.mmd_schwarz.argtypes = [_ctx_p, _vp]
+lib.mmd_overlap_powers.restype = _c_int
+lib.mmd_overlap_powers.argtypes = [_ctx_p, _vp, _vp, _vp, ctypes.POINTER(_c_int)]
+lib.mmd_dgemm.restype = _c_int
+lib.mmd_dgemm.argtypes = [_ctx_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_double, _vp, _c_int, _vp, _c_int,
+                          _c_double, _vp, _c_int, _c_int]
 lib.mmd_set_lds_limit.restype = _c_int
 lib.mmd_set_lds_limit.argtypes = [_ctx_p, _c_i64]
 lib.mmd_set_schwarz.restype = _c_int
@@ -114,7 +119,7 @@ EXPORTED = [
     "mmd_set_basis", "mmd_set_atoms", "mmd_one_electron", "mmd_schwarz", "mmd_set_schwarz",
     "mmd_eri_tensor", "mmd_eri_list", "mmd_fock_delta", "mmd_fock_delta_noalloc", "mmd_jk_tensor", "mmd_boys",
     "mmd_last_fock_ms", "mmd_set_profiling", "mmd_fock_stats", "mmd_set_precision", "mmd_fock_tiers",
-    "mmd_set_streams", "mmd_set_lds_limit", "mmd_scf_begin", "mmd_scf_step", "mmd_scf_density", "mmd_scf_get", "mmd_ao2mo",
+    "mmd_set_streams", "mmd_set_lds_limit", "mmd_overlap_powers", "mmd_dgemm", "mmd_scf_begin", "mmd_scf_step", "mmd_scf_density", "mmd_scf_get", "mmd_ao2mo",
     "mmd_mlp_create", "mmd_mlp_destroy", "mmd_mlp_nparam", "mmd_mlp_set_params", "mmd_mlp_get_params",
     "mmd_mlp_forward", "mmd_mlp_loss_grad", "mmd_mlp_train",
 ]
@@ -202,6 +207,28 @@ class Context:
         M, L = np.empty((3, N, N)), np.empty((3, N, N))
         check(lib.mmd_one_electron(self._h, _ptr(c), _ptr(S), _ptr(T), _ptr(V), _ptr(M), _ptr(L)))
         return S, T, V, M, L
+
+    def overlap_powers(self, S=None):
+        """(X, U, iterations): S^-1/2 and S^1/2 on the GPU (Newton-Schulz on
+        the matrix cores); S None uses the overlap of the last one_electron."""
+        N = self.nbf
+        X, U = np.empty((N, N)), np.empty((N, N))
+        it = _c_int(0)
+        src = None if S is None else _f64(S).reshape(N, N)
+        check(lib.mmd_overlap_powers(self._h, None if src is None else _ptr(src), _ptr(X), _ptr(U),
+                                     ctypes.byref(it)))
+        return X, U, it.value
+
+    def dgemm(self, A, B, transA=False, transB=False, alpha=1.0, beta=0.0, C=None):
+        """alpha op(A) op(B) + beta C on the matrix cores (host arrays)."""
+        A, B = _f64(A), _f64(B)
+        M = A.shape[1] if transA else A.shape[0]
+        K = A.shape[0] if transA else A.shape[1]
+        N = B.shape[0] if transB else B.shape[1]
+        Cm = np.zeros((M, N)) if C is None else np.array(C, dtype=np.float64, order="C")
+        check(lib.mmd_dgemm(self._h, int(transA), int(transB), M, N, K, float(alpha), _ptr(A), A.shape[1],
+                            _ptr(B), B.shape[1], float(beta), _ptr(Cm), max(1, N), 0))
+        return Cm
 
     def schwarz(self):
         d = np.empty((self.nbf, self.nbf))

@@ -232,10 +232,9 @@ class Molecule(SCF):
             self.nuc_energy += a.charge * b.charge / np.linalg.norm(a.origin - b.origin)
         self.Core = self.T + self.V
         # S^(-1/2) and S^(1/2), the principal powers of the symmetric positive
-        # definite overlap (fractional_matrix_power in mmd/molecule.py:285-286)
-        s, V = np.linalg.eigh(self.S)
-        self.X = np.dot(V * s ** -0.5, V.T)
-        self.U = np.dot(V * s ** 0.5, V.T)
+        # definite overlap (fractional_matrix_power in mmd/molecule.py:285-286),
+        # on the GPU from the overlap still resident there
+        self.X, self.U, _ = self.native.overlap_powers()
         assert self.S.shape == (N, N)
 
     def two_electron_integrals(self):

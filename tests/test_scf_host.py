@@ -26,6 +26,12 @@ class OracleBackedContext:
         S, T, V, M, L = self.om.one_electron()
         return S, T, V, M, L
 
+    def overlap_powers(self, S=None):
+        if S is None:
+            S = self.om.one_electron()[0]
+        s, V = np.linalg.eigh(S)
+        return (V * s ** -0.5) @ V.T, (V * s ** 0.5) @ V.T, 0
+
     def schwarz(self):
         N = self.nbf
         d = np.empty((N, N))

@@ -401,7 +401,7 @@ int build_pairs(mmd_ctx* c, const std::vector<double>* qpair)
         c->qc_total[qc] = (pcb == pck) ? nb * (nb + 1) / 2 : nb * nk;
         c->qc_off[qc + 1] = c->qc_off[qc] + c->qc_total[qc];
     }
-    HIP_TRY(c->d_count.reserve(NQCLASS));
+    HIP_TRY(c->d_count.reserve(2 * NQCLASS));
     c->pairs_ok = true;
     c->units_ok = false;
     for (int qc = 0; qc < NQCLASS; ++qc) c->plan_ms[qc] = 0.0;
@@ -501,7 +501,7 @@ ClassArgs base_args(mmd_ctx* c)
 // caller-free static array (mmd_fock_delta_noalloc).
 struct FockScratch {
     int* rowctr = nullptr;                 // 2 * NQCLASS
-    int* count = nullptr;                  // NQCLASS
+    int* count = nullptr;                  // 2 * NQCLASS: survivor-list lengths (FP64 lists, then FP32)
     unsigned long long* nprim = nullptr;   // 3 * NQCLASS
     double* Dsh = nullptr;                 // nshell^2
     int2* surv = nullptr;
@@ -536,11 +536,12 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol, FockScratch& W0)
                          (kj_lds[qc] || fock_class_lds(qc, 0, c->nbf, maxpp[qc], 0) <= c->max_lds);
     }
     {
-        // survivor lists only for the classes that use them, packed
+        // survivor lists only for the classes that use them, packed; with the
+        // FP32 tier each class has a second list right after the first
         int64_t off = 0;
         for (int qc = 0; qc < NQCLASS; ++qc) {
             W0.surv_off[qc] = off;
-            if (c->qc_total[qc] && !row_kernel[qc]) off += c->qc_total[qc];
+            if (c->qc_total[qc] && !row_kernel[qc]) off += (tier32 ? 2 : 1) * c->qc_total[qc];
         }
         if (W0.surv_cap < 0) {
             HIP_TRY(c->d_surv.reserve((size_t)std::max<int64_t>(off, 1)));
@@ -551,7 +552,7 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol, FockScratch& W0)
         }
     }
     HIP_TRY(launch_fill_int(2 * NQCLASS, W0.rowctr, 0, c->st));
-    HIP_TRY(launch_fill_int(NQCLASS, W0.count, 0, c->st));
+    HIP_TRY(launch_fill_int(2 * NQCLASS, W0.count, 0, c->st));
     HIP_TRY(launch_fill_int(6 * NQCLASS, reinterpret_cast<int*>(W0.nprim), 0, c->st));
     // Classes run concurrently on the main and side streams (longest first,
     // each to the least loaded stream), so that the small classes fill the
@@ -646,15 +647,26 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol, FockScratch& W0)
                 HIP_TRY(launch_fock_class(qc, 1, A, R, grid, sq));
             }
         } else {
+            // one screening pass; with the FP32 tier the quartets with bound
+            // in [tol, fp32_below) go to a second list and run in FP32
             int2* surv = W0.surv + W0.surv_off[qc];
+            int2* surv32 = tier32 ? surv + c->qc_total[qc] : nullptr;
             HIP_TRY(launch_screen_class(c->pc_off[pcb], nb, c->pc_off[pck], nk, pcb == pck ? 1 : 0, c->d_pairs.p,
-                                        W0.Dsh, (int)c->shells.size(), tol, 0, surv, W0.count + qc, sq));
+                                        W0.Dsh, (int)c->shells.size(), tol, 0, surv, W0.count + qc, sq,
+                                        c->fp32_below, surv32, W0.count + NQCLASS + qc));
             ClassArgs W = A;
             W.quart = surv;
             W.count = W0.count + qc;
             if (c->profiling)
                 HIP_TRY(launch_count_prims(W.quart, W.count, c->d_pairs.p, W0.nprim + qc, sq));
-            HIP_TRY(launch_fock_wave_class(la, lb, lc, ld, W, c->qc_total[qc], sq));
+            HIP_TRY(launch_fock_wave_class(la, lb, lc, ld, 0, W, c->qc_total[qc], sq));
+            if (tier32) {
+                W.quart = surv32;
+                W.count = W0.count + NQCLASS + qc;
+                if (c->profiling)
+                    HIP_TRY(launch_count_prims(W.quart, W.count, c->d_pairs.p, W0.nprim + qc, sq));
+                HIP_TRY(launch_fock_wave_class(la, lb, lc, ld, 1, W, c->qc_total[qc], sq));
+            }
         }
         if (events) HIP_TRY(hipEventRecord(c->evc[2 * qc + 1], sq));
     }
@@ -671,14 +683,15 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol, FockScratch& W0)
 // Called after the build's end event, so the readback is not timed.
 int collect_profile(mmd_ctx* c)
 {
-    int h[NQCLASS];
+    int h[2 * NQCLASS];
     unsigned long long hp[3 * NQCLASS];
-    HIP_TRY(hipMemcpyAsync(h, c->d_count.p, sizeof(int) * NQCLASS, hipMemcpyDeviceToHost, c->st));
+    HIP_TRY(hipMemcpyAsync(h, c->d_count.p, sizeof(int) * 2 * NQCLASS, hipMemcpyDeviceToHost, c->st));
     HIP_TRY(hipMemcpyAsync(hp, c->d_nprim.p, sizeof(hp), hipMemcpyDeviceToHost, c->st));
     HIP_TRY(hipStreamSynchronize(c->st));
     for (int qc = 0; qc < NQCLASS; ++qc) {
         c->st_nquart[qc] = c->qc_total[qc]
-                               ? (c->last_row_kernel[qc] ? (int64_t)(hp[NQCLASS + qc] + hp[2 * NQCLASS + qc]) : h[qc])
+                               ? (c->last_row_kernel[qc] ? (int64_t)(hp[NQCLASS + qc] + hp[2 * NQCLASS + qc])
+                                                         : (int64_t)h[qc] + h[NQCLASS + qc])
                                : 0;
         c->st_nprim[qc] = c->qc_total[qc] ? (int64_t)hp[qc] : 0;
         float ms = 0.f;
@@ -980,7 +993,7 @@ hipError_t eri_tensor_shells(mmd_ctx* c, double* dst)
         const int koff = c->pc_off[pck], nk = c->pc_off[pck + 1] - koff;
         if (nb == 0 || nk == 0) continue;
         const int tri = pcb == pck;
-        if (fock_class_is_wave(la, lb, lc, ld)) {
+        if (la == 2 && lb == 2 && lc == 2 && ld == 2) {   // (dd|dd): the wavefront tensor kernel
             std::vector<int2> q;
             q.reserve((size_t)nb * (nb + 1) / 2);
             for (int P = 0; P < nb; ++P)
@@ -1103,15 +1116,15 @@ int mmd_fock_delta(mmd_ctx* ctx, const double* dP_re, const double* dP_im, doubl
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->st));
     if (!raw) {
         // survivors: per-class lists (wave classes) + FP64 and FP32 row-kernel counts
-        int h[NQCLASS];
+        int h[2 * NQCLASS];
         unsigned long long hq[2 * NQCLASS];
-        HIP_TRY(hipMemcpyAsync(h, ctx->d_count.p, sizeof(int) * NQCLASS, hipMemcpyDeviceToHost, ctx->st));
+        HIP_TRY(hipMemcpyAsync(h, ctx->d_count.p, sizeof(int) * 2 * NQCLASS, hipMemcpyDeviceToHost, ctx->st));
         HIP_TRY(hipMemcpyAsync(hq, ctx->d_nprim.p + NQCLASS, sizeof(hq), hipMemcpyDeviceToHost, ctx->st));
         HIP_TRY(hipStreamSynchronize(ctx->st));
         int64_t s = 0, s32 = 0;
         for (int i = 0; i < NQCLASS; ++i) {
-            s += (int64_t)h[i] + (int64_t)hq[i] + (int64_t)hq[NQCLASS + i];
-            s32 += (int64_t)hq[NQCLASS + i];
+            s += (int64_t)h[i] + (int64_t)h[NQCLASS + i] + (int64_t)hq[i] + (int64_t)hq[NQCLASS + i];
+            s32 += (int64_t)h[NQCLASS + i] + (int64_t)hq[NQCLASS + i];
         }
         ctx->last_nsurv = s;
         ctx->last_nsurv_fp32 = s32;
@@ -1167,7 +1180,7 @@ int mmd_fock_delta_noalloc(mmd_ctx* ctx, const double* dP, double tol, double* G
     int* rowctr = reinterpret_cast<int*>(base + off);
     off += up(sizeof(int) * 2 * NQCLASS);
     int* count = reinterpret_cast<int*>(base + off);
-    off += up(sizeof(int) * NQCLASS);
+    off += up(sizeof(int) * 2 * NQCLASS);
     unsigned long long* nprim = reinterpret_cast<unsigned long long*>(base + off);
     off += up(sizeof(unsigned long long) * 3 * NQCLASS);
     if (off > NOALLOC_SCRATCH_BYTES)
@@ -1195,15 +1208,15 @@ int mmd_fock_delta_noalloc(mmd_ctx* ctx, const double* dP, double tol, double* G
     if (rc) return rc;
     HIP_TRY(launch_symmetrize(N, Gacc, G, ctx->st));
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->st));
-    int h[NQCLASS];
+    int h[2 * NQCLASS];
     unsigned long long hq[2 * NQCLASS];
-    HIP_TRY(hipMemcpyAsync(h, count, sizeof(int) * NQCLASS, hipMemcpyDeviceToHost, ctx->st));
+    HIP_TRY(hipMemcpyAsync(h, count, sizeof(int) * 2 * NQCLASS, hipMemcpyDeviceToHost, ctx->st));
     HIP_TRY(hipMemcpyAsync(hq, nprim + NQCLASS, sizeof(hq), hipMemcpyDeviceToHost, ctx->st));
     HIP_TRY(hipStreamSynchronize(ctx->st));
     int64_t sum = 0, s32 = 0;
     for (int i = 0; i < NQCLASS; ++i) {
-        sum += (int64_t)h[i] + (int64_t)hq[i] + (int64_t)hq[NQCLASS + i];
-        s32 += (int64_t)hq[NQCLASS + i];
+        sum += (int64_t)h[i] + (int64_t)h[NQCLASS + i] + (int64_t)hq[i] + (int64_t)hq[NQCLASS + i];
+        s32 += (int64_t)h[NQCLASS + i] + (int64_t)hq[NQCLASS + i];
     }
     ctx->last_nsurv = sum;
     ctx->last_nsurv_fp32 = s32;

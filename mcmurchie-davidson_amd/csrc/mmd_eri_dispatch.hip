@@ -62,6 +62,26 @@ hipError_t launch_fock_class(int qc, int fp32, const ClassArgs& A, const RowArgs
     return hipErrorInvalidValue;  // wavefront-per-quartet classes are launched separately
 }
 
+#define MMD_MFMA_CLASSES(X) X(2, 2, 2, 2) X(2, 2, 2, 1) X(2, 2, 1, 1) X(2, 2, 2, 0) X(2, 1, 2, 1) X(2, 1, 1, 1) X(2, 1, 2, 0)
+
+#define DECLM(a, b, c, d)                                                                                    \
+    hipError_t fock_mfma_launch_##a##b##c##d##_d(const ClassArgs& A, int64_t max_quartets, hipStream_t st); \
+    hipError_t fock_mfma_launch_##a##b##c##d##_f(const ClassArgs& A, int64_t max_quartets, hipStream_t st);
+MMD_MFMA_CLASSES(DECLM)
+#undef DECLM
+
+hipError_t launch_fock_mfma_class(int la, int lb, int lc, int ld, int fp32, const ClassArgs& A, int64_t max_quartets,
+                                  hipStream_t st)
+{
+#define CASE(a, b, c, d)                                                                    \
+    if (la == a && lb == b && lc == c && ld == d)                                           \
+        return fp32 ? fock_mfma_launch_##a##b##c##d##_f(A, max_quartets, st)                \
+                    : fock_mfma_launch_##a##b##c##d##_d(A, max_quartets, st);
+    MMD_MFMA_CLASSES(CASE)
+#undef CASE
+    return hipErrorInvalidValue;
+}
+
 size_t fock_class_lds(int qc, int fp32, int N, int maxpp, int kj)
 {
     int la, lb, lc, ld;

@@ -26,7 +26,12 @@ inline __host__ __device__ void qclass_decode(int qc, int* la, int* lb, int* lc,
 hipError_t launch_fock_class(int qc, int fp32, const ClassArgs& A, const RowArgs& R, int grid, hipStream_t st);
 // kj = 1: including the ket-side Coulomb buffer of the classes that have one.
 size_t fock_class_lds(int qc, int fp32, int N, int maxpp, int kj);
-hipError_t launch_fock_wave_class(int la, int lb, int lc, int ld, const ClassArgs& A, int64_t max_quartets,
+// Classes whose shell quartets run one per wavefront from survivor lists:
+// the high angular-momentum classes on the matrix cores (fp32 = 1: FP32
+// MFMA), any other class on the generic FP64 wavefront kernel.
+hipError_t launch_fock_wave_class(int la, int lb, int lc, int ld, int fp32, const ClassArgs& A, int64_t max_quartets,
+                                  hipStream_t st);
+hipError_t launch_fock_mfma_class(int la, int lb, int lc, int ld, int fp32, const ClassArgs& A, int64_t max_quartets,
                                   hipStream_t st);
 // Stored ERI tensor from shell quartets: class kernels (every class but
 // (dd|dd)) and the wavefront kernel for (dd|dd).  table: staged bra data,
@@ -35,8 +40,15 @@ hipError_t launch_tensor_class(int qc, const ClassArgs& A, int boff, int nb, int
                                double* table, double* out, hipStream_t st);
 int tensor_class_bps(int qc);
 hipError_t launch_tensor_wave_class(const ClassArgs& A, int64_t nquartets, hipStream_t st);
-// Classes served by the wavefront-per-quartet kernel instead of a unrolled class kernel.
-inline bool fock_class_is_wave(int la, int lb, int lc, int ld) { return la == 2 && lb == 2 && lc == 2 && ld == 2; }
+// High angular-momentum classes whose two Hermite contractions run on the
+// matrix cores (fock_mfma_class_kernel), one wavefront per shell quartet.
+inline bool fock_class_is_mfma(int la, int lb, int lc, int ld)
+{
+    return la == 2 && ((lb == 2 && lc + ld >= 2) || (lb == 1 && lc == 2) || (lb == 1 && lc == 1 && ld == 1));
+}
+// Classes served by a wavefront-per-quartet kernel (survivor lists) instead of
+// the unrolled row kernel.
+inline bool fock_class_is_wave(int la, int lb, int lc, int ld) { return fock_class_is_mfma(la, lb, lc, ld); }
 
 // A single Cartesian basis function for the generic (per basis-function
 // quartet) kernel: centre, exponents (l,m,n), primitive range; the
@@ -59,9 +71,11 @@ hipError_t launch_one_electron(int nbf, const BFDesc* bfs, const double* pexp, c
                                const double* atoms /* natom x 4: Z, x, y, z */, double cx, double cy, double cz,
                                const double* boys, double* S, double* T, double* V, double* M, double* Lm,
                                hipStream_t st);
+// Survivor list of one class; with surv2, the quartets whose bound lies in
+// [tol, split) go to the second list instead (the FP32 tier).
 hipError_t launch_screen_class(int boff, int nb, int koff, int nk, int tri, const ShellPair* pairs,
                                const double* Dsh, int nshell, double tol, int keep_all, int2* surv, int* count,
-                               hipStream_t st);
+                               hipStream_t st, double split = 0.0, int2* surv2 = nullptr, int* count2 = nullptr);
 hipError_t launch_abs_density(int n, const double* Dre, const double* Dim, double* absD, hipStream_t st);
 hipError_t launch_shell_block_max(int nshell, const int* sh_bf0, const int* sh_nbf, int N, const double* absD,
                                   double* Dsh, hipStream_t st);

@@ -373,7 +373,8 @@ hipError_t launch_one_electron(int nbf, const BFDesc* bfs, const double* pexp, c
 __global__ void __launch_bounds__(256) screen_class_kernel(int boff, int nb, int koff, int nk, int tri,
                                                            const ShellPair* __restrict__ pairs,
                                                            const double* __restrict__ Dsh, int nshell, double tol,
-                                                           int keep_all, int2* __restrict__ surv, int* count)
+                                                           int keep_all, int2* __restrict__ surv, int* count,
+                                                           double split, int2* __restrict__ surv2, int* count2)
 {
     const int64_t total = tri ? (int64_t)nb * (nb + 1) / 2 : (int64_t)nb * nk;
     const int lane = threadIdx.x & 63;
@@ -381,7 +382,7 @@ __global__ void __launch_bounds__(256) screen_class_kernel(int boff, int nb, int
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
     for (int64_t base = wave0 * 64; base < total; base += nwaves * 64) {
         const int64_t idx = base + lane;
-        bool keep = false;
+        bool keep = false, low = false;
         int P = 0, Q = 0;
         if (idx < total) {
             if (tri) {
@@ -402,30 +403,42 @@ __global__ void __launch_bounds__(256) screen_class_kernel(int boff, int nb, int
                 const double dm = fmax(fmax(fmax(4.0 * Dsh[bra.sa * ns + bra.sb], 4.0 * Dsh[ket.sa * ns + ket.sb]),
                                             fmax(Dsh[bra.sa * ns + ket.sa], Dsh[bra.sa * ns + ket.sb])),
                                        fmax(Dsh[bra.sb * ns + ket.sa], Dsh[bra.sb * ns + ket.sb]));
-                keep = bra.q * ket.q * dm >= tol;
+                const double bound = bra.q * ket.q * dm;
+                keep = bound >= tol;
+                // with a second list: the quartets below `split` go there
+                low = keep && surv2 && bound < split;
             }
         }
-        const unsigned long long m = __ballot(keep);
+        const unsigned long long m = __ballot(keep && !low);
         const int cnt = __popcll(m);
         int pos = 0;
         if (lane == 0 && cnt) pos = atomicAdd(count, cnt);
         pos = __shfl(pos, 0);
-        if (keep) {
+        if (keep && !low) {
             const int off = __popcll(m & ((1ull << lane) - 1ull));
             surv[pos + off] = make_int2(boff + P, koff + Q);
+        }
+        if (surv2) {
+            const unsigned long long m2 = __ballot(low);
+            const int cnt2 = __popcll(m2);
+            int pos2 = 0;
+            if (lane == 0 && cnt2) pos2 = atomicAdd(count2, cnt2);
+            pos2 = __shfl(pos2, 0);
+            if (low) surv2[pos2 + __popcll(m2 & ((1ull << lane) - 1ull))] = make_int2(boff + P, koff + Q);
         }
     }
 }
 
 hipError_t launch_screen_class(int boff, int nb, int koff, int nk, int tri, const ShellPair* pairs, const double* Dsh,
-                               int nshell, double tol, int keep_all, int2* surv, int* count, hipStream_t st)
+                               int nshell, double tol, int keep_all, int2* surv, int* count, hipStream_t st,
+                               double split, int2* surv2, int* count2)
 {
     const int64_t total = tri ? (int64_t)nb * (nb + 1) / 2 : (int64_t)nb * nk;
     if (total <= 0) return hipSuccess;
     int64_t grid = (total + 255) / 256;
     if (grid > 8192) grid = 8192;
     hipLaunchKernelGGL(screen_class_kernel, dim3((unsigned)grid), dim3(256), 0, st, boff, nb, koff, nk, tri, pairs,
-                       Dsh, nshell, tol, keep_all, surv, count);
+                       Dsh, nshell, tol, keep_all, surv, count, split, surv2, count2);
     return hipGetLastError();
 }
 
@@ -753,9 +766,10 @@ __global__ void __launch_bounds__(64 * GW_WAVES) fock_wave_class_kernel_t(ClassA
     }
 }
 
-hipError_t launch_fock_wave_class(int la, int lb, int lc, int ld, const ClassArgs& A, int64_t max_quartets,
+hipError_t launch_fock_wave_class(int la, int lb, int lc, int ld, int fp32, const ClassArgs& A, int64_t max_quartets,
                                   hipStream_t st)
 {
+    if (fock_class_is_mfma(la, lb, lc, ld)) return launch_fock_mfma_class(la, lb, lc, ld, fp32, A, max_quartets, st);
     int64_t grid = (max_quartets + GW_WAVES - 1) / GW_WAVES;
     if (grid < 1) grid = 1;
     if (grid > 4096) grid = 4096;

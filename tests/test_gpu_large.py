@@ -164,3 +164,27 @@ def test_h2o16_fp32_tier_energy(native):
     assert ref.is_converged and mix.is_converged
     assert n32 > 0
     assert abs(mix.energy.real - ref.energy.real) < 1e-6
+
+
+def test_fp32_matrix_core_classes_close_to_fp64(native):
+    """With every quartet in the FP32 tier (threshold 1), the FP32 class
+    kernels (FP32 MFMA for the high-L classes, FP32 VALU for the others)
+    stay within FP32 rounding of the FP64 build: 1e-5 of max |G|."""
+    from mmd.molecule import Molecule
+    mol = Molecule(water_cluster(2, 7.0, 0), "cc-pvdz")
+    mol.build(direct=True)
+    ctx = mol.native
+    rng = np.random.default_rng(9)
+    dP = _sym_random(mol.nbasis, rng, 0.1)
+    G64, _, n64 = ctx.fock_delta(dP, None, tol=1e-12)
+    ctx.set_precision(1.0)
+    try:
+        G32, _, n32 = ctx.fock_delta(dP, None, tol=1e-12)
+        ntot, nfp32 = ctx.fock_tiers()
+    finally:
+        ctx.set_precision(0.0)
+    err = np.max(np.abs(G32 - G64))
+    print("all-FP32: %d of %d shell quartets in FP32, max |dG| %.2e of max |G| %.2e"
+          % (nfp32, ntot, err, np.abs(G64).max()))
+    assert n32 == n64 and nfp32 > 0.9 * ntot
+    assert err < 1e-5 * np.abs(G64).max()

@@ -65,6 +65,17 @@ int mmd_set_basis(mmd_ctx *ctx, int nbf, const double *origin, const int32_t *lm
 int mmd_set_atoms(mmd_ctx *ctx, int natom, const double *Z, const double *xyz);
 
 /*
+ * Several molecules in one basis, block-diagonally: the basis of
+ * mmd_set_basis and the nuclei of mmd_set_atoms are those of nmol molecules
+ * one after the other, molecule m owning nbf_per_mol[m] functions and
+ * natom_per_mol[m] nuclei.  Integrals, pairs and quartets never mix two
+ * molecules: every N x N result is block-diagonal, and one Fock build serves
+ * all molecules.  No counterpart in the reference (one Molecule at a time);
+ * used by mmd_batch_rhf.
+ */
+int mmd_set_molecules(mmd_ctx *ctx, int nmol, const int32_t *nbf_per_mol, const int32_t *natom_per_mol);
+
+/*
  * One-electron integrals (Molecule.one_electron_integrals,
  * mmd/molecule.py:235-281, with S/T/V/Mu/RxDel of cython/onee.pyx:14-69):
  * S, T, V (sum over nuclei of -Z V_C): N*N each; M (dipole, origin `center`)
@@ -225,6 +236,32 @@ int mmd_scf_begin(mmd_ctx *ctx, int nocc, const double *H, const double *S, cons
 int mmd_scf_step(mmd_ctx *ctx, double tol, int build, int diis, double *FO);
 int mmd_scf_density(mmd_ctx *ctx, const double *CO, double *stats);
 int mmd_scf_get(mmd_ctx *ctx, double *F, double *P, double *P_old, double *G);
+
+/*
+ * Direct RHF of every molecule of the context's basis at once (the loop of
+ * SCF.RHF, mmd/scf.py:10-80 and 146-180, with direct=True), all on the GPU:
+ * one-electron integrals, Schwarz screen and X = S^-1/2 per molecule, then
+ * per iteration one Fock build for the whole batch (the class kernels) and
+ * one workgroup per molecule for the F update, DIIS (window 8),
+ * FO = X^T F X, a Jacobi eigensolver, the density, the energy and the test
+ * ||P - P_old||_F < conver.  A converged molecule keeps its state and adds no
+ * work to later builds.  The iteration is captured once and replayed as a
+ * HIP graph when use_graph = 1.  No counterpart in the reference.
+ *   nocc, enuc        : per molecule (nmol of mmd_set_molecules)
+ *   P0                : NULL (core guess) or per-molecule initial densities,
+ *                       n_m x n_m each, packed one after the other
+ *   conver, acc2e     : as SCF.RHF; maxiter: the reference's 100
+ *   diis              : 1 as SCF.RHF(DIIS=True)
+ *   energy, iterations, converged : per molecule outputs (iterations: the
+ *                       step index of convergence, as SCF.iterations)
+ *   P_out (nullable)  : converged densities, packed as P0
+ *   mo_out (nullable) : orbital energies, N (molecule after molecule)
+ * Molecules of up to 88 basis functions.  The FP32 tier of mmd_set_precision
+ * applies to the Fock builds.
+ */
+int mmd_batch_rhf(mmd_ctx *ctx, const int32_t *nocc, const double *enuc, const double *P0, double conver,
+                  double acc2e, int maxiter, int diis, int use_graph, double *energy, int32_t *iterations,
+                  int32_t *converged, double *P_out, double *mo_out);
 
 /*
  * Four-index transformation of the stored ERI tensor to the molecular-orbital

@@ -17,6 +17,7 @@
 
 #include "../../include/mmd.h"
 #include <rocblas/rocblas.h>
+#include "mmd_batch.h"
 #include "mmd_device.h"
 #include "mmd_gemm.h"
 #include "mmd_kernels.h"
@@ -51,6 +52,7 @@ int fail(int code, const char* fmt, ...)
 
 struct Shell {
     int l, np, p0, bf0;
+    int mol;   // molecule of the shell's functions
     double x, y, z;
 };
 
@@ -169,6 +171,21 @@ struct mmd_ctx {
     int natom = 0;
     DevBuf<double> d_atoms;
 
+    // Molecules of a block-diagonal basis (mmd_set_molecules; one molecule
+    // unless set): molecule m owns functions [mol_f0[m], mol_f0[m] + mol_n[m])
+    // and shells [mol_s0[m], mol_s0[m + 1]); pairs and quartets never mix
+    // molecules.  rw: the largest molecule (width of the class kernels' LDS
+    // rows); plist: the same-molecule function pairs p >= q.
+    int nmol = 1;
+    std::vector<int> mol_f0{0}, mol_n{0}, mol_s0{0, 0};
+    std::vector<int> atom_mol;
+    DevBuf<int> d_atom_mol;
+    DevBuf<int2> d_plist;
+    int64_t npl = 0;
+    int rw = 0;
+    // per pair class, per molecule: the class-local range [first, end) of its pairs
+    std::vector<int> kr_first[NPAIRCLASS], kr_end[NPAIRCLASS];
+
     // density / Fock scratch
     DevBuf<double> d_D, d_Dim, d_absD, d_Dsh, d_G, d_Gout;
 
@@ -194,6 +211,13 @@ struct mmd_ctx {
     size_t max_lds_device = 65536;      // the device's limit per workgroup
     bool last_row_kernel[NQCLASS] = {};
     int64_t st_nquart[NQCLASS] = {0};
+
+    // batched RHF (mmd_batch_rhf): N x N block-diagonal matrices, per-molecule
+    // work, DIIS windows and state
+    DevBuf<double> b_H, b_X, b_U, b_F, b_P, b_Pold, b_dP, b_G, b_work, b_diis, b_mo;
+    DevBuf<BatchMol> b_mol;
+    DevBuf<BatchState> b_state;
+    DevBuf<int> b_flag, b_count;
 
     // GPU-resident SCF loop (mmd_scf_*): matrices, DIIS window, BLAS handle
     rocblas_handle blas = nullptr;
@@ -255,6 +279,7 @@ void build_shells(mmd_ctx* c)
         }
         Shell s;
         s.l = L;
+        s.mol = b0.pad;
         s.np = b0.np;
         s.p0 = (int)c->sh_exp.size();
         s.bf0 = i;
@@ -306,80 +331,119 @@ constexpr double PRIM_DROP = 1e-20;
 // Shell pairs I >= J in canonical orientation, with primitive-pair data.
 // `qpair` (optional): Schwarz factors; pairs are then sorted inside each
 // pair class by (npp desc, q desc).
-int build_pairs(mmd_ctx* c, const std::vector<double>* qpair)
+// Shells [mol_s0[m], mol_s0[m+1]) of each molecule (shells never span two).
+void molecule_shell_ranges(mmd_ctx* c)
 {
     const int ns = (int)c->shells.size();
+    c->mol_s0.assign(c->nmol + 1, ns);
+    for (int I = ns - 1; I >= 0; --I) c->mol_s0[c->shells[I].mol] = I;
+    for (int m = c->nmol - 1; m >= 0; --m)
+        if (c->mol_s0[m] > c->mol_s0[m + 1]) c->mol_s0[m] = c->mol_s0[m + 1];
+}
+
+// Shell pairs of each molecule, grouped by pair class and sorted in each
+// class by (molecule, primitive pairs descending, Schwarz factor
+// descending), with their primitive-pair tables.  diag: the full N x N
+// Schwarz diagonal (pq|pq), or null before it is known (factors 1).
+int build_pairs(mmd_ctx* c, const std::vector<double>* diag)
+{
+    const int N = c->nbf;
+    molecule_shell_ranges(c);
     // largest primitive-pair bound, for the primitive-pair screen
     c->prim_qmax = 0.0;
-    for (int I = 0; I < ns; ++I)
-        for (int J = 0; J <= I; ++J) {
-            const Shell& A = c->shells[I];
-            const Shell& B = c->shells[J];
-            const double AB2 = (A.x - B.x) * (A.x - B.x) + (A.y - B.y) * (A.y - B.y) + (A.z - B.z) * (A.z - B.z);
-            for (int i = 0; i < A.np; ++i)
-                for (int j = 0; j < B.np; ++j) {
-                    const double ea = c->sh_exp[A.p0 + i], eb = c->sh_exp[B.p0 + j];
-                    const double p = ea + eb;
-                    const double K = c->sh_w[A.p0 + i] * c->sh_w[B.p0 + j] * exp(-ea * eb / p * AB2);
-                    c->prim_qmax = std::max(c->prim_qmax, prim_bound(p, K, AB2, A.l + B.l));
-                }
-        }
+    for (int m = 0; m < c->nmol; ++m)
+        for (int I = c->mol_s0[m]; I < c->mol_s0[m + 1]; ++I)
+            for (int J = c->mol_s0[m]; J <= I; ++J) {
+                const Shell& A = c->shells[I];
+                const Shell& B = c->shells[J];
+                const double AB2 = (A.x - B.x) * (A.x - B.x) + (A.y - B.y) * (A.y - B.y) + (A.z - B.z) * (A.z - B.z);
+                for (int i = 0; i < A.np; ++i)
+                    for (int j = 0; j < B.np; ++j) {
+                        const double ea = c->sh_exp[A.p0 + i], eb = c->sh_exp[B.p0 + j];
+                        const double p = ea + eb;
+                        const double K = c->sh_w[A.p0 + i] * c->sh_w[B.p0 + j] * exp(-ea * eb / p * AB2);
+                        c->prim_qmax = std::max(c->prim_qmax, prim_bound(p, K, AB2, A.l + B.l));
+                    }
+            }
     std::vector<std::vector<ShellPair>> byclass(NPAIRCLASS);
     std::vector<PrimPair> pp;
-    int pidx = 0;
-    for (int I = 0; I < ns; ++I)
-        for (int J = 0; J <= I; ++J, ++pidx) {
-            int a = I, b = J;
-            if (c->shells[I].l < c->shells[J].l) std::swap(a, b);
-            const Shell& A = c->shells[a];
-            const Shell& B = c->shells[b];
-            ShellPair sp;
-            sp.sa = a;
-            sp.sb = b;
-            sp.bfa = A.bf0;
-            sp.bfb = B.bf0;
-            sp.pp0 = (int)pp.size();
-            sp.same = (a == b) ? 1 : 0;
-            sp.pad = pidx;
-            sp.q = qpair ? (*qpair)[pidx] : 1.0;
-            const double AB2 = (A.x - B.x) * (A.x - B.x) + (A.y - B.y) * (A.y - B.y) + (A.z - B.z) * (A.z - B.z);
-            for (int i = 0; i < A.np; ++i)
-                for (int j = 0; j < B.np; ++j) {
-                    const double ea = c->sh_exp[A.p0 + i], eb = c->sh_exp[B.p0 + j];
-                    PrimPair q;
-                    q.p = ea + eb;
-                    q.K = c->sh_w[A.p0 + i] * c->sh_w[B.p0 + j] * exp(-ea * eb / q.p * AB2);
-                    // Primitive-pair screening (Fock-build tables only): a pair
-                    // whose Schwarz-type bound, times the largest bound of any
-                    // pair, is below PRIM_DROP contributes less than that to
-                    // every primitive quartet and is left out.
-                    if (prim_bound(q.p, q.K, AB2, A.l + B.l) * c->prim_qmax < PRIM_DROP) continue;
-                    q.Px = (ea * A.x + eb * B.x) / q.p;
-                    q.Py = (ea * A.y + eb * B.y) / q.p;
-                    q.Pz = (ea * A.z + eb * B.z) / q.p;
-                    q.PAx = q.Px - A.x;
-                    q.PAy = q.Py - A.y;
-                    q.PAz = q.Pz - A.z;
-                    q.PBx = q.Px - B.x;
-                    q.PBy = q.Py - B.y;
-                    q.PBz = q.Pz - B.z;
-                    q.inv2p = 0.5 / q.p;
-                    q.invp = 1.0 / q.p;
-                    pp.push_back(q);
+    for (int m = 0; m < c->nmol; ++m)
+        for (int I = c->mol_s0[m]; I < c->mol_s0[m + 1]; ++I)
+            for (int J = c->mol_s0[m]; J <= I; ++J) {
+                int a = I, b = J;
+                if (c->shells[I].l < c->shells[J].l) std::swap(a, b);
+                const Shell& A = c->shells[a];
+                const Shell& B = c->shells[b];
+                ShellPair sp;
+                sp.sa = a;
+                sp.sb = b;
+                sp.bfa = A.bf0;
+                sp.bfb = B.bf0;
+                sp.pp0 = (int)pp.size();
+                sp.same = (a == b) ? 1 : 0;
+                sp.mol = m;
+                sp.c0 = c->mol_f0[m];
+                sp.pad = 0;
+                // pair factor q = max sqrt((ab|ab)) over the pair's functions
+                double q = 1.0;
+                if (diag) {
+                    q = 0.0;
+                    for (int x = 0; x < ncart(A.l); ++x)
+                        for (int y = 0; y < ncart(B.l); ++y) {
+                            const double v = (*diag)[(size_t)(A.bf0 + x) * N + B.bf0 + y];
+                            const double r = v > 0.0 ? sqrt(v) : 0.0;
+                            if (r > q) q = r;
+                        }
                 }
-            sp.npp = (int)pp.size() - sp.pp0;
-            if (sp.npp == 0) sp.q = 0.0;   // no primitive pair survives: never screened in
-            byclass[pair_class(A.l, B.l)].push_back(sp);
-        }
+                sp.q = q;
+                const double AB2 = (A.x - B.x) * (A.x - B.x) + (A.y - B.y) * (A.y - B.y) + (A.z - B.z) * (A.z - B.z);
+                for (int i = 0; i < A.np; ++i)
+                    for (int j = 0; j < B.np; ++j) {
+                        const double ea = c->sh_exp[A.p0 + i], eb = c->sh_exp[B.p0 + j];
+                        PrimPair pq;
+                        pq.p = ea + eb;
+                        pq.K = c->sh_w[A.p0 + i] * c->sh_w[B.p0 + j] * exp(-ea * eb / pq.p * AB2);
+                        // Primitive-pair screening (Fock-build tables only): a pair
+                        // whose Schwarz-type bound, times the largest bound of any
+                        // pair, is below PRIM_DROP contributes less than that to
+                        // every primitive quartet and is left out.
+                        if (prim_bound(pq.p, pq.K, AB2, A.l + B.l) * c->prim_qmax < PRIM_DROP) continue;
+                        pq.Px = (ea * A.x + eb * B.x) / pq.p;
+                        pq.Py = (ea * A.y + eb * B.y) / pq.p;
+                        pq.Pz = (ea * A.z + eb * B.z) / pq.p;
+                        pq.PAx = pq.Px - A.x;
+                        pq.PAy = pq.Py - A.y;
+                        pq.PAz = pq.Pz - A.z;
+                        pq.PBx = pq.Px - B.x;
+                        pq.PBy = pq.Py - B.y;
+                        pq.PBz = pq.Pz - B.z;
+                        pq.inv2p = 0.5 / pq.p;
+                        pq.invp = 1.0 / pq.p;
+                        pp.push_back(pq);
+                    }
+                sp.npp = (int)pp.size() - sp.pp0;
+                if (sp.npp == 0) sp.q = 0.0;   // no primitive pair survives: never screened in
+                byclass[pair_class(A.l, B.l)].push_back(sp);
+            }
     c->pairs.clear();
     for (int pc = 0; pc < NPAIRCLASS; ++pc) {
         auto& v = byclass[pc];
-        if (qpair)
+        if (diag)
             std::stable_sort(v.begin(), v.end(), [](const ShellPair& x, const ShellPair& y) {
+                if (x.mol != y.mol) return x.mol < y.mol;
                 if (x.npp != y.npp) return x.npp > y.npp;
                 return x.q > y.q;
             });
         c->pc_off[pc] = (int)c->pairs.size();
+        c->kr_first[pc].assign(c->nmol, (int)v.size());
+        c->kr_end[pc].assign(c->nmol, 0);
+        for (int k = 0; k < (int)v.size(); ++k) {
+            const int m = v[k].mol;
+            c->kr_first[pc][m] = std::min(c->kr_first[pc][m], k);
+            c->kr_end[pc][m] = std::max(c->kr_end[pc][m], k + 1);
+        }
+        for (int m = 0; m < c->nmol; ++m)
+            if (c->kr_first[pc][m] > c->kr_end[pc][m]) c->kr_first[pc][m] = c->kr_end[pc][m] = 0;
         c->pairs.insert(c->pairs.end(), v.begin(), v.end());
     }
     c->pc_off[NPAIRCLASS] = (int)c->pairs.size();
@@ -390,7 +454,8 @@ int build_pairs(mmd_ctx* c, const std::vector<double>* qpair)
                            c->st));
     HIP_TRY(hipMemcpyAsync(c->d_pp.p, pp.data(), sizeof(PrimPair) * pp.size(), hipMemcpyHostToDevice, c->st));
     HIP_TRY(hipStreamSynchronize(c->st));
-    // quartet-class sizes (implicit enumeration)
+    // quartet-class sizes (implicit enumeration; for several molecules an
+    // upper bound, the screen keeps the quartets of one molecule)
     c->qc_off[0] = 0;
     for (int qc = 0; qc < NQCLASS; ++qc) {
         int pcb = 0;
@@ -434,20 +499,25 @@ int build_units(mmd_ctx* c)
         // make a class that runs alone faster, but cost more than they gain
         // when classes or molecules share the GPU.)
         int seg = 4096;
-        while (seg > 256 && quartets / seg < 4096) seg /= 2;
+        while (seg > 256 && quartets / std::max(1, c->nmol) / seg < 4096) seg /= 2;
         c->unit_seg[qc] = seg;
         if (nb == 0 || nk == 0) continue;
         kpre.assign(nk + 1, 0.0);
         for (int k = 0; k < nk; ++k) kpre[k + 1] = kpre[k] + c->pairs[koff + k].npp;
         v.clear();
         for (int P = 0; P < nb; ++P) {
-            const int npp = c->pairs[boff + P].npp;
-            const int nkP = tri ? P + 1 : nk;
-            for (int k0 = 0; k0 < nkP; k0 += seg) {
-                const double kc = kpre[std::min(nkP, k0 + seg)] - kpre[k0];
+            const ShellPair& bp = c->pairs[boff + P];
+            const int npp = bp.npp;
+            // the kets of the bra pair's molecule (up to the bra pair itself
+            // in the triangular classes)
+            const int kfirst = c->kr_first[pck][bp.mol];
+            const int kend = tri ? P + 1 : c->kr_end[pck][bp.mol];
+            for (int k0 = kfirst; k0 < kend; k0 += seg) {
+                const int k1 = std::min(kend, k0 + seg);
+                const double kc = kpre[k1] - kpre[k0];
                 for (int ip = 0; ip < npp; ip += PSLICE) {
                     const int ip1 = std::min(npp, ip + PSLICE);
-                    v.push_back({(ip1 - ip) * kc, make_int4(P, ip, ip1, k0)});
+                    v.push_back({(ip1 - ip) * kc, make_int4(P, ip | (ip1 << 16), k0, k1)});
                 }
             }
         }
@@ -531,9 +601,9 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol, FockScratch& W0)
         for (int P = c->pc_off[pcb]; P < c->pc_off[pcb + 1]; ++P) maxpp[qc] = std::max(maxpp[qc], c->pairs[P].npp);
         // the ket-side Coulomb buffer where it fits; the row kernel without it
         // where only that fits; else the wavefront-per-quartet kernel
-        kj_lds[qc] = fock_class_lds(qc, 0, c->nbf, maxpp[qc], 1) <= c->max_lds;
+        kj_lds[qc] = fock_class_lds(qc, 0, c->rw, maxpp[qc], 1) <= c->max_lds;
         row_kernel[qc] = !fock_class_is_wave(la, lb, lc, ld) &&
-                         (kj_lds[qc] || fock_class_lds(qc, 0, c->nbf, maxpp[qc], 0) <= c->max_lds);
+                         (kj_lds[qc] || fock_class_lds(qc, 0, c->rw, maxpp[qc], 0) <= c->max_lds);
     }
     {
         // survivor lists only for the classes that use them, packed; with the
@@ -625,6 +695,7 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol, FockScratch& W0)
             R.maxpp = maxpp[qc];
             R.kj = kj_lds[qc] ? 1 : 0;
             R.seg = c->unit_seg[qc];
+            R.rw = c->rw;
             R.units = c->d_units.p + c->unit_off[qc];
             const int64_t units = c->unit_off[qc + 1] - c->unit_off[qc];
             R.nunits = (int)units;
@@ -715,25 +786,7 @@ int set_schwarz(mmd_ctx* ctx, const std::vector<double>& h)
     HIP_TRY(hipStreamSynchronize(ctx->st));
     ctx->schwarz_ok = true;
     if (!ctx->shells_ok) return MMD_OK;   // only the per-function paths are available
-    // pair factors q = max sqrt((ab|ab)) over the pair's functions, indexed by
-    // the pair's position in the I >= J enumeration
-    const int ns = (int)ctx->shells.size();
-    std::vector<double> q((size_t)ns * (ns + 1) / 2, 0.0);
-    int pidx = 0;
-    for (int I = 0; I < ns; ++I)
-        for (int J = 0; J <= I; ++J, ++pidx) {
-            const Shell& A1 = ctx->shells[I];
-            const Shell& B1 = ctx->shells[J];
-            double m = 0.0;
-            for (int a = 0; a < ncart(A1.l); ++a)
-                for (int b = 0; b < ncart(B1.l); ++b) {
-                    const double x = h[(size_t)(A1.bf0 + a) * N + B1.bf0 + b];
-                    const double s = x > 0.0 ? sqrt(x) : 0.0;
-                    if (s > m) m = s;
-                }
-            q[pidx] = m;
-        }
-    const int rc = build_pairs(ctx, &q);
+    const int rc = build_pairs(ctx, &h);   // pair factors q from the diagonal
     if (rc) return rc;
     // the Fock build's work units belong to the pair table: built here, at
     // set-up, so that a build (mmd_fock_delta_noalloc) allocates nothing
@@ -879,6 +932,12 @@ int mmd_set_basis(mmd_ctx* ctx, int nbf, const double* origin, const int32_t* lm
     ctx->nbf = nbf;
     ctx->pairs_ok = false;
     ctx->schwarz_ok = false;
+    ctx->nmol = 1;
+    ctx->mol_f0.assign(1, 0);
+    ctx->mol_n.assign(1, nbf);
+    ctx->atom_mol.clear();
+    ctx->npl = 0;
+    ctx->rw = nbf;
     HIP_TRY(ctx->d_bf.reserve(nbf));
     HIP_TRY(ctx->d_bf_exp.reserve(ctx->bf_exp.size()));
     HIP_TRY(ctx->d_bf_w.reserve(ctx->bf_w.size()));
@@ -924,6 +983,60 @@ int mmd_set_atoms(mmd_ctx* ctx, int natom, const double* Z, const double* xyz)
     return MMD_OK;
 }
 
+int mmd_set_molecules(mmd_ctx* ctx, int nmol, const int32_t* nbf_per_mol, const int32_t* natom_per_mol)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (ctx->nbf == 0) return fail(MMD_ERR_STATE, "no basis set");
+    if (nmol < 1 || !nbf_per_mol || !natom_per_mol) return fail(MMD_ERR_ARG, "invalid molecule arguments");
+    int64_t sf = 0, sa = 0;
+    for (int m = 0; m < nmol; ++m) {
+        if (nbf_per_mol[m] < 1 || natom_per_mol[m] < 0) return fail(MMD_ERR_ARG, "molecule %d is empty", m);
+        sf += nbf_per_mol[m];
+        sa += natom_per_mol[m];
+    }
+    if (sf != ctx->nbf) return fail(MMD_ERR_ARG, "molecule sizes add up to %lld functions, the basis has %d",
+                                    (long long)sf, ctx->nbf);
+    if (sa != ctx->natom) return fail(MMD_ERR_ARG, "molecule atom counts add up to %lld, %d atoms are set",
+                                      (long long)sa, ctx->natom);
+    ctx->nmol = nmol;
+    ctx->mol_f0.assign(nmol, 0);
+    ctx->mol_n.assign(nmol, 0);
+    ctx->rw = 0;
+    std::vector<int2> pl;
+    for (int m = 0, f = 0; m < nmol; ++m) {
+        ctx->mol_f0[m] = f;
+        ctx->mol_n[m] = nbf_per_mol[m];
+        ctx->rw = std::max(ctx->rw, nbf_per_mol[m]);
+        for (int i = f; i < f + nbf_per_mol[m]; ++i) {
+            ctx->bf[i].pad = m;
+            for (int j = f; j <= i; ++j) pl.push_back(make_int2(i, j));
+        }
+        f += nbf_per_mol[m];
+    }
+    ctx->atom_mol.clear();
+    for (int m = 0; m < nmol; ++m)
+        for (int a = 0; a < natom_per_mol[m]; ++a) ctx->atom_mol.push_back(m);
+    ctx->npl = (int64_t)pl.size();
+    HIP_TRY(hipMemcpyAsync(ctx->d_bf.p, ctx->bf.data(), sizeof(BFDesc) * ctx->nbf, hipMemcpyHostToDevice, ctx->st));
+    HIP_TRY(ctx->d_plist.reserve(pl.size()));
+    HIP_TRY(hipMemcpyAsync(ctx->d_plist.p, pl.data(), sizeof(int2) * pl.size(), hipMemcpyHostToDevice, ctx->st));
+    HIP_TRY(ctx->d_atom_mol.reserve(std::max<size_t>(1, ctx->atom_mol.size())));
+    if (!ctx->atom_mol.empty())
+        HIP_TRY(hipMemcpyAsync(ctx->d_atom_mol.p, ctx->atom_mol.data(), sizeof(int) * ctx->atom_mol.size(),
+                               hipMemcpyHostToDevice, ctx->st));
+    HIP_TRY(hipStreamSynchronize(ctx->st));
+    build_shells(ctx);
+    if (ctx->shells_ok) {
+        rc = upload_shells(ctx);
+        if (rc) return rc;
+    }
+    ctx->pairs_ok = false;
+    ctx->schwarz_ok = false;
+    ctx->units_ok = false;
+    return MMD_OK;
+}
+
 int mmd_one_electron(mmd_ctx* ctx, const double center[3], double* S, double* T, double* V, double* M, double* L)
 {
     int rc = check_ctx(ctx);
@@ -933,9 +1046,11 @@ int mmd_one_electron(mmd_ctx* ctx, const double center[3], double* S, double* T,
     const size_t nn = (size_t)ctx->nbf * ctx->nbf;
     DevBuf<double>& d = ctx->d_onee;   // kept between calls (no allocation per SCF)
     HIP_TRY(d.reserve(9 * nn));
+    const bool several = ctx->nmol > 1;
     HIP_TRY(launch_one_electron(ctx->nbf, ctx->d_bf.p, ctx->d_bf_exp.p, ctx->d_bf_w.p, ctx->natom,
                                 ctx->natom ? ctx->d_atoms.p : nullptr, center[0], center[1], center[2], ctx->d_boys.p,
-                                d.p, d.p + nn, d.p + 2 * nn, d.p + 3 * nn, d.p + 6 * nn, ctx->st));
+                                d.p, d.p + nn, d.p + 2 * nn, d.p + 3 * nn, d.p + 6 * nn, ctx->st,
+                                several ? ctx->d_plist.p : nullptr, ctx->npl, several ? ctx->d_atom_mol.p : nullptr));
     HIP_TRY(hipMemcpyAsync(S, d.p, sizeof(double) * nn, hipMemcpyDeviceToHost, ctx->st));
     HIP_TRY(hipMemcpyAsync(T, d.p + nn, sizeof(double) * nn, hipMemcpyDeviceToHost, ctx->st));
     HIP_TRY(hipMemcpyAsync(V, d.p + 2 * nn, sizeof(double) * nn, hipMemcpyDeviceToHost, ctx->st));
@@ -954,7 +1069,8 @@ int mmd_schwarz(mmd_ctx* ctx, double* diag)
     const size_t nn = (size_t)N * N;
     std::vector<double> h(nn);
     HIP_TRY(ctx->d_schwarz.reserve(nn));
-    HIP_TRY(launch_eri_diag(N, ctx->d_bf.p, ctx->d_bf_exp.p, ctx->d_bf_w.p, ctx->d_boys.p, ctx->d_schwarz.p, ctx->st));
+    HIP_TRY(launch_eri_diag(N, ctx->d_bf.p, ctx->d_bf_exp.p, ctx->d_bf_w.p, ctx->d_boys.p, ctx->d_schwarz.p, ctx->st,
+                            ctx->nmol > 1 ? ctx->d_plist.p : nullptr, ctx->npl));
     HIP_TRY(hipMemcpyAsync(h.data(), ctx->d_schwarz.p, sizeof(double) * nn, hipMemcpyDeviceToHost, ctx->st));
     HIP_TRY(hipStreamSynchronize(ctx->st));
     if (diag) memcpy(diag, h.data(), sizeof(double) * nn);
@@ -1387,6 +1503,248 @@ int scf_check(mmd_ctx* c)
 }  // namespace
 
 extern "C" {
+
+namespace {
+// Enqueue G = sym(G(dP)) (the symmetrised build of mmd_fock_delta) on the
+// context's stream: no allocation and no host synchronisation, so that a
+// batched iteration can be captured into a graph.
+int enqueue_fock_sym(mmd_ctx* c, const double* dP, double tol, double* Gout)
+{
+    const int N = c->nbf;
+    const size_t nn = (size_t)N * N;
+    HIP_TRY(launch_abs_density((int)nn, dP, nullptr, c->d_absD.p, c->st));
+    HIP_TRY(launch_shell_block_max((int)c->shells.size(), c->d_sh_bf0.p, c->d_sh_nbf.p, N, c->d_absD.p, c->d_Dsh.p,
+                                   c->st));
+    ClassArgs A = base_args(c);
+    A.absD = c->d_absD.p;
+    A.sqS = c->d_sqS.p;
+    A.tol = tol;
+    A.D = dP;
+    A.G = c->d_G.p;
+    HIP_TRY(launch_fill(nn, c->d_G.p, 0.0, c->st));
+    FockScratch W;
+    W.rowctr = c->d_rowctr.p;
+    W.count = c->d_count.p;
+    W.nprim = c->d_nprim.p;
+    W.Dsh = c->d_Dsh.p;
+    W.surv = c->d_surv.p;
+    W.surv_cap = (int64_t)c->d_surv.n;   // reserved by the caller
+    W.allow_measure = false;
+    int rc = run_fock_classes(c, A, tol, W);
+    if (rc) return rc;
+    HIP_TRY(launch_symmetrize(N, c->d_G.p, Gout, c->st));
+    return MMD_OK;
+}
+
+__global__ void batch_axpy_kernel(size_t n, const double* __restrict__ a, const double* __restrict__ b,
+                                  double* __restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a[i] + b[i];
+}
+}  // namespace
+
+int mmd_batch_rhf(mmd_ctx* ctx, const int32_t* nocc, const double* enuc, const double* P0, double conver,
+                  double acc2e, int maxiter, int diis, int use_graph, double* energy, int32_t* iterations,
+                  int32_t* converged, double* P_out, double* mo_out)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    mmd_ctx* c = ctx;
+    if (c->nbf == 0) return fail(MMD_ERR_STATE, "no basis set");
+    if (!c->shells_ok) return fail(MMD_ERR_UNSUPPORTED, "%s", c->shells_err.c_str());
+    if (!nocc || !enuc || !energy || !iterations || !converged || maxiter < 1)
+        return fail(MMD_ERR_ARG, "invalid batch arguments");
+    const int M = c->nmol, N = c->nbf;
+    const size_t nn = (size_t)N * N;
+    std::vector<BatchMol> mols(M);
+    size_t wtot = 0, dtot = 0, ptot = 0;
+    for (int m = 0; m < M; ++m) {
+        const int n = c->mol_n[m];
+        if (n > BATCH_MAX_NBF)
+            return fail(MMD_ERR_UNSUPPORTED, "molecule %d has %d basis functions; the batch takes up to %d", m, n,
+                        BATCH_MAX_NBF);
+        if (nocc[m] < 1 || nocc[m] > n) return fail(MMD_ERR_ARG, "molecule %d: invalid occupation", m);
+        BatchMol& b = mols[m];
+        b.f0 = c->mol_f0[m];
+        b.n = n;
+        b.nocc = nocc[m];
+        b.pad = 0;
+        b.wo = wtot;
+        b.d8 = dtot;
+        b.enuc = enuc[m];
+        wtot += 3 * (size_t)n * n;
+        dtot += 16 * (size_t)n * n;
+        ptot += (size_t)n * n;
+    }
+    hipStream_t st = c->st;
+    // one-electron integrals of every molecule's own block; H = T + V
+    HIP_TRY(c->d_onee.reserve(9 * nn));
+    double* S = c->d_onee.p;
+    HIP_TRY(launch_one_electron(N, c->d_bf.p, c->d_bf_exp.p, c->d_bf_w.p, c->natom,
+                                c->natom ? c->d_atoms.p : nullptr, 0.0, 0.0, 0.0, c->d_boys.p, S, S + nn, S + 2 * nn,
+                                S + 3 * nn, S + 6 * nn, st, M > 1 ? c->d_plist.p : nullptr, c->npl,
+                                M > 1 ? c->d_atom_mol.p : nullptr));
+    for (DevBuf<double>* b : {&c->b_H, &c->b_X, &c->b_U, &c->b_F, &c->b_P, &c->b_Pold, &c->b_dP, &c->b_G})
+        HIP_TRY(b->reserve(nn));
+    HIP_TRY(c->b_work.reserve(std::max<size_t>(1, wtot)));
+    HIP_TRY(c->b_diis.reserve(std::max<size_t>(1, dtot)));
+    HIP_TRY(c->b_mo.reserve(N));
+    HIP_TRY(c->b_mol.reserve(M));
+    HIP_TRY(c->b_state.reserve(M));
+    HIP_TRY(c->b_flag.reserve(M));
+    HIP_TRY(c->b_count.reserve(1));
+    hipLaunchKernelGGL(batch_axpy_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, nn, S + nn, S + 2 * nn,
+                       c->b_H.p);
+    HIP_TRY(hipGetLastError());
+    // Schwarz diagonal, pair tables and work units of the whole batch
+    HIP_TRY(c->d_schwarz.reserve(nn));
+    HIP_TRY(launch_eri_diag(N, c->d_bf.p, c->d_bf_exp.p, c->d_bf_w.p, c->d_boys.p, c->d_schwarz.p, st,
+                            M > 1 ? c->d_plist.p : nullptr, c->npl));
+    {
+        std::vector<double> h(nn);
+        HIP_TRY(hipMemcpyAsync(h.data(), c->d_schwarz.p, sizeof(double) * nn, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        if ((rc = set_schwarz(c, h))) return rc;
+    }
+    // buffers of the Fock build, reserved before any capture
+    HIP_TRY(c->d_rowctr.reserve(2 * NQCLASS));
+    HIP_TRY(c->d_nprim.reserve(3 * NQCLASS));
+    {
+        int64_t wave = 0;
+        for (int qc = 0; qc < NQCLASS; ++qc) {
+            int la, lb, lc, ld;
+            qclass_decode(qc, &la, &lb, &lc, &ld);
+            if (c->qc_total[qc] && fock_class_is_wave(la, lb, lc, ld)) wave += 2 * c->qc_total[qc];
+        }
+        // survivor lists of the wavefront classes; row-kernel fallbacks need
+        // more and are refused by the capacity check
+        HIP_TRY(c->d_surv.reserve((size_t)std::max<int64_t>(wave, 1)));
+    }
+    // state: core guess F = H, P = P_old = 0; or the given densities
+    std::vector<BatchState> st0(M);
+    for (auto& x : st0) {
+        memset(&x, 0, sizeof(x));
+        x.energy = 0.0;
+        x.prms = 1e20;
+    }
+    HIP_TRY(hipMemcpyAsync(c->b_mol.p, mols.data(), sizeof(BatchMol) * M, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->b_state.p, st0.data(), sizeof(BatchState) * M, hipMemcpyHostToDevice, st));
+    HIP_TRY(launch_fill_int(M, c->b_flag.p, 0, st));
+    HIP_TRY(hipMemcpyAsync(c->b_F.p, c->b_H.p, sizeof(double) * nn, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(launch_fill(nn, c->b_P.p, 0.0, st));
+    BatchArgs B;
+    B.nmol = M;
+    B.N = N;
+    B.mol = c->b_mol.p;
+    B.state = c->b_state.p;
+    B.S = S;
+    B.H = c->b_H.p;
+    B.X = c->b_X.p;
+    B.U = c->b_U.p;
+    B.F = c->b_F.p;
+    B.P = c->b_P.p;
+    B.Pold = c->b_Pold.p;
+    B.dP = c->b_dP.p;
+    B.G = c->b_G.p;
+    B.work = c->b_work.p;
+    B.diis = c->b_diis.p;
+    B.mo = c->b_mo.p;
+    B.flag = c->b_flag.p;
+    HIP_TRY(launch_batch_overlap(B, st));
+    if (P0) {
+        // an initial density (SAD, learned): F = H + G(P0), P_old = P0
+        std::vector<double> full(nn, 0.0);
+        size_t o = 0;
+        for (int m = 0; m < M; ++m) {
+            const int n = c->mol_n[m], f0 = c->mol_f0[m];
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < n; ++j) full[(size_t)(f0 + i) * N + f0 + j] = P0[o + (size_t)i * n + j];
+            o += (size_t)n * n;
+        }
+        HIP_TRY(hipMemcpyAsync(c->b_P.p, full.data(), sizeof(double) * nn, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        if ((rc = enqueue_fock_sym(c, c->b_P.p, acc2e, c->b_G.p))) return rc;
+        hipLaunchKernelGGL(batch_axpy_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, nn, c->b_H.p,
+                           c->b_G.p, c->b_F.p);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipMemcpyAsync(c->b_Pold.p, c->b_P.p, sizeof(double) * nn, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(launch_batch_step(B, 0, diis, conver, st));
+    {
+        std::vector<int> fl(M);
+        HIP_TRY(hipMemcpyAsync(fl.data(), c->b_flag.p, sizeof(int) * M, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        for (int m = 0; m < M; ++m)
+            if (fl[m] == -1) return fail(MMD_ERR_ARG, "molecule %d: overlap matrix not positive definite", m);
+    }
+    // iterations 1 .. maxiter-1: Fock build of the whole batch from dP, then
+    // the per-molecule step; replayed from a graph when use_graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    auto iteration = [&]() -> int {
+        int r = enqueue_fock_sym(c, c->b_dP.p, acc2e, c->b_G.p);
+        if (r) return r;
+        HIP_TRY(launch_batch_step(B, 1, diis, conver, st));
+        return MMD_OK;
+    };
+    if (use_graph) {
+        if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) == hipSuccess) {
+            const int r = iteration();
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(st, &g);
+            if (r == MMD_OK && e == hipSuccess && g &&
+                hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) == hipSuccess) {
+                graph = g;
+            } else {
+                if (g) (void)hipGraphDestroy(g);
+                exec = nullptr;
+                (void)hipGetLastError();
+                if (r) return r;
+            }
+        }
+    }
+    int left = M;
+    for (int it = 1; it < maxiter && left > 0; ++it) {
+        if (exec) {
+            HIP_TRY(hipGraphLaunch(exec, st));
+        } else if ((rc = iteration())) {
+            return rc;
+        }
+        if (it % 4 == 0 || it == maxiter - 1) {
+            HIP_TRY(launch_batch_count(B, c->b_count.p, st));
+            HIP_TRY(hipMemcpyAsync(&left, c->b_count.p, sizeof(int), hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+        }
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    std::vector<BatchState> out(M);
+    std::vector<int> fl(M);
+    HIP_TRY(hipMemcpyAsync(out.data(), c->b_state.p, sizeof(BatchState) * M, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(fl.data(), c->b_flag.p, sizeof(int) * M, hipMemcpyDeviceToHost, st));
+    if (mo_out) HIP_TRY(hipMemcpyAsync(mo_out, c->b_mo.p, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+    std::vector<double> full;
+    if (P_out) {
+        full.resize(nn);
+        HIP_TRY(hipMemcpyAsync(full.data(), c->b_P.p, sizeof(double) * nn, hipMemcpyDeviceToHost, st));
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    size_t o = 0;
+    for (int m = 0; m < M; ++m) {
+        if (fl[m] == -2) return fail(MMD_ERR_STATE, "molecule %d: singular DIIS matrix", m);
+        energy[m] = out[m].energy;
+        iterations[m] = out[m].iterations;
+        converged[m] = out[m].converged;
+        if (P_out) {
+            const int n = c->mol_n[m], f0 = c->mol_f0[m];
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < n; ++j) P_out[o + (size_t)i * n + j] = full[(size_t)(f0 + i) * N + f0 + j];
+            o += (size_t)n * n;
+        }
+    }
+    return MMD_OK;
+}
 
 int mmd_scf_begin(mmd_ctx* ctx, int nocc, const double* H, const double* S, const double* X, const double* F,
                   const double* P)

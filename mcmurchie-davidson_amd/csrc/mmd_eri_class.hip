@@ -21,15 +21,15 @@ namespace mmd {
 hipError_t MMD_NAME(fock_class_launch_, QC_LA, QC_LB, QC_LC, QC_LD, QC_SUFFIX)(const ClassArgs& A, const RowArgs& R,
                                                                                int grid, hipStream_t st)
 {
-    const size_t lds = fock_class_lds_bytes<QC_LA, QC_LB, QC_LC, QC_LD, QC_T>(A.N, R.maxpp, R.kj);
+    const size_t lds = fock_class_lds_bytes<QC_LA, QC_LB, QC_LC, QC_LD, QC_T>(R.rw, R.maxpp, R.kj);
     hipLaunchKernelGGL((eri_fock_class_kernel<QC_LA, QC_LB, QC_LC, QC_LD, QC_T>), dim3(grid), dim3(CLASS_BLOCK), lds,
                        st, A, R);
     return hipGetLastError();
 }
 
-size_t MMD_NAME(fock_class_lds_, QC_LA, QC_LB, QC_LC, QC_LD, QC_SUFFIX)(int N, int maxpp, int kj)
+size_t MMD_NAME(fock_class_lds_, QC_LA, QC_LB, QC_LC, QC_LD, QC_SUFFIX)(int rw, int maxpp, int kj)
 {
-    return fock_class_lds_bytes<QC_LA, QC_LB, QC_LC, QC_LD, QC_T>(N, maxpp, kj);
+    return fock_class_lds_bytes<QC_LA, QC_LB, QC_LC, QC_LD, QC_T>(rw, maxpp, kj);
 }
 
 #ifndef QC_F32

@@ -70,6 +70,13 @@ struct ClassShape {
     // and digest each quartet once; QB is their batch size (buffer <= 16 KB).
     static constexpr int NABCD = NAB * NCD;
     static constexpr bool IBUF = NABCD <= 32;
+    // Items are (quartet, ket primitive slice) pairs, not whole quartets,
+    // for the buffered classes and for the classes whose ket pairs are the
+    // heavily contracted (ss) and (ps) pairs (up to 81 primitive pairs per
+    // ket in cc-pVDZ): there a whole-ket lane would carry up to 81 primitive
+    // quartets alone and set the length of its batch.  Unbuffered classes
+    // digest each slice's partial block directly (the digestion is linear).
+    static constexpr bool KSL = IBUF || (LC <= 1 && LD == 0);
     // The other classes digest each (quartet, chunk) item; when a quartet
     // has several chunks, their ket-side Coulomb blocks are summed in an LDS
     // buffer per batch (KJ) and added to G once per quartet.
@@ -219,12 +226,14 @@ __device__ __forceinline__ void quartet_compute(const ClassArgs& A, const T* lds
 }
 
 // Digest the integrals acc of a quartet (bra components A0..A1-1) into G.
-// rowA / rowB: rows of G for the bra shells' functions (row x of shell a at
-// rowA + x*N) accumulated in LDS; the ket-side Coulomb block goes to G.
+// rowA / rowB: rows of G for the bra shells' functions accumulated in LDS,
+// row x holding column `col` of G at rowA[x * RW + col] (the caller offsets
+// the pointers by the first column of the molecule); the ket-side Coulomb
+// block goes to G.
 template <int LA, int LB, int LC, int LD, int A0, int A1>
 __device__ __forceinline__ void quartet_digest(const ClassArgs& A, int2 qd, const ShellPair& bra,
                                                const ShellPair& ket, double* acc, double* rowA, double* rowB,
-                                               double* kj = nullptr)
+                                               int RW, double* kj = nullptr)
 {
     using S = ClassShape<LA, LB, LC, LD>;
     constexpr int NCH = A1 - A0;
@@ -277,7 +286,7 @@ __device__ __forceinline__ void quartet_digest(const ClassArgs& A, int2 qd, cons
             constexpr int c = cd / S::ND, d = cd % S::ND;
             s = fma(acc[AI(ab, cd)], D[(size_t)(bfc + c) * N + bfd + d], s);
         });
-        atomicAdd(&rowA[(size_t)a * N + bfb + b], s);
+        atomicAdd(&rowA[a * RW + bfb + b], s);
     });
     // Coulomb: G_cd += sum_ab (ab|cd) D_ab   (ket block: to G, or to the
     // quartet's LDS row kj when given, added to G after the batch)
@@ -315,7 +324,7 @@ __device__ __forceinline__ void quartet_digest(const ClassArgs& A, int2 qd, cons
                     });
                 }
             });
-            if (any) atomicAdd(&rowB[(size_t)x * N + bfd + y], -0.25 * s);
+            if (any) atomicAdd(&rowB[x * RW + bfd + y], -0.25 * s);
         });
         sfor<S::NC>([&](auto iy) {        // (b,c) block
             constexpr int y = decltype(iy)::value;
@@ -332,7 +341,7 @@ __device__ __forceinline__ void quartet_digest(const ClassArgs& A, int2 qd, cons
                     });
                 }
             });
-            if (any) atomicAdd(&rowB[(size_t)x * N + bfc + y], -0.25 * s);
+            if (any) atomicAdd(&rowB[x * RW + bfc + y], -0.25 * s);
         });
     });
     sfor<S::NA>([&](auto ix) {
@@ -352,7 +361,7 @@ __device__ __forceinline__ void quartet_digest(const ClassArgs& A, int2 qd, cons
                     });
                 }
             });
-            if (any) atomicAdd(&rowA[(size_t)x * N + bfc + y], -0.25 * s);
+            if (any) atomicAdd(&rowA[x * RW + bfc + y], -0.25 * s);
         });
         sfor<S::ND>([&](auto iy) {        // (a,d) block
             constexpr int y = decltype(iy)::value;
@@ -369,7 +378,7 @@ __device__ __forceinline__ void quartet_digest(const ClassArgs& A, int2 qd, cons
                     });
                 }
             });
-            if (any) atomicAdd(&rowA[(size_t)x * N + bfd + y], -0.25 * s);
+            if (any) atomicAdd(&rowA[x * RW + bfd + y], -0.25 * s);
         });
     });
 }
@@ -416,7 +425,8 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
     static_assert(QB <= CLASS_BLOCK, "one thread per batch entry in the slice-count scan");
     extern __shared__ double dyn[];
     double* rows = dyn;
-    double* ibuf = rows + (S::NA + S::NB) * A.N;
+    const int RW = R.rw;
+    double* ibuf = rows + (S::NA + S::NB) * RW;
     // ket-side Coulomb buffer (KJ classes, when the launch has room for it)
     const bool kj = S::KJ && R.kj;
     const int nbuf = S::IBUF_DOUBLES + (kj ? S::KJ_DOUBLES : 0);
@@ -449,19 +459,20 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
             }
             break;
         }
-        // unit: bra pair P, its primitive pairs [U.y, U.z) and the ket
-        // segment starting at U.w (k0 < the row's ket count)
+        // unit: bra pair P, its primitive pairs [ip0, ip1) and the ket
+        // pairs [k0, k1) of its molecule
         const int4 U = R.units[u];
-        const int P = U.x, k0 = U.w;
-        const int nk = R.tri ? P + 1 : R.nk;
-        const int segn = min(R.seg, nk - k0);
-        const int bnp = U.z - U.y;   // bra primitive pairs of the unit (<= PSLICE)
+        const int P = U.x, k0 = U.z;
+        const int ip0u = U.y & 0xffff, ip1u = U.y >> 16;
+        const int segn = U.w - U.z;
+        const int bnp = ip1u - ip0u;   // bra primitive pairs of the unit (<= PSLICE)
         const ShellPair bra = A.pairs[R.boff + P];
         const int nrows = bra.same ? S::NA : S::NA + S::NB;
-        double* rowA = rows;
-        double* rowB = rows + (bra.same ? 0 : S::NA * N);
-        for (int i = threadIdx.x; i < nrows * N; i += blockDim.x) rows[i] = 0.0;
-        stage_bra<LA, LB, LC, LD, T>(A, bra, U.y, bnp, bp);
+        const int c0 = bra.c0;       // the rows hold the molecule's columns c0 .. c0+RW-1
+        double* rowA = rows - c0;
+        double* rowB = rows + (bra.same ? 0 : S::NA * RW) - c0;
+        for (int i = threadIdx.x; i < nrows * RW; i += blockDim.x) rows[i] = 0.0;
+        stage_bra<LA, LB, LC, LD, T>(A, bra, ip0u, bnp, bp);
         if (threadIdx.x == 0) misc[1] = 0;   // survivors in the list
         __syncthreads();
         for (int t0 = 0;; t0 += (int)blockDim.x) {
@@ -500,7 +511,7 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
                 const int cnt = min(QB, cnt0 - b0);
                 if (R.stat_nquart && threadIdx.x == 0) atomicAdd(R.stat_nquart, (unsigned long long)cnt);
                 int nflat = cnt;
-                if constexpr (S::IBUF) {
+                if constexpr (S::KSL) {
                     const int j = threadIdx.x;   // cnt <= QB <= CLASS_BLOCK
                     int x = 0;
                     if (j < cnt) x = (A.pairs[R.koff + qlist[b0 + j]].npp + PSLICE - 1) / PSLICE;
@@ -522,7 +533,7 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
                     const int flat = (w / S::NCHUNK) * 64 + lane;
                     if (flat >= nflat) continue;
                     int idx = flat, ks = 0;
-                    if constexpr (S::IBUF) {
+                    if constexpr (S::KSL) {
                         // the quartet whose slices cover item flat: pre[idx] <= flat < pre[idx + 1]
                         int lo = 0, hi = cnt - 1;
                         while (lo < hi) {
@@ -538,7 +549,7 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
                     const ShellPair ket = A.pairs[qd.y];
                     const int jq0 = ks * PSLICE;
                     if (jq0 >= ket.npp) continue;
-                    const int jq1 = S::IBUF ? min(ket.npp, jq0 + PSLICE) : ket.npp;
+                    const int jq1 = S::KSL ? min(ket.npp, jq0 + PSLICE) : ket.npp;
                     const int ip0 = 0, ip1 = bnp;
                     if (chunk == 0) my_nprim += (unsigned long long)(ip1 - ip0) * (unsigned long long)(jq1 - jq0);
                     static_switch<0, S::NCHUNK>(chunk, [&](auto ic) {
@@ -553,7 +564,7 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
                         } else {
                             double accd[(A1 - A0) * S::NCD];
                             sfor<(A1 - A0) * S::NCD>([&](auto k) { accd[k] = (double)acc[k]; });
-                            quartet_digest<LA, LB, LC, LD, A0, A1>(A, qd, bra, ket, accd, rowA, rowB,
+                            quartet_digest<LA, LB, LC, LD, A0, A1>(A, qd, bra, ket, accd, rowA, rowB, RW,
                                                                    kj ? ibuf + (size_t)idx * S::NCD : nullptr);
                         }
                     });
@@ -591,7 +602,7 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
                                 acc[k] = src[decltype(k)::value];
                                 src[decltype(k)::value] = 0.0;
                             });
-                            quartet_digest<LA, LB, LC, LD, A0, A1>(A, qd, bra, ket, acc, rowA, rowB);
+                            quartet_digest<LA, LB, LC, LD, A0, A1>(A, qd, bra, ket, acc, rowA, rowB, RW);
                         });
                     }
                 }
@@ -611,9 +622,9 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
             __syncthreads();
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < nrows * N; i += blockDim.x) {
+        for (int i = threadIdx.x; i < nrows * RW; i += blockDim.x) {
             const double v = rows[i];
-            const int x = i / N, col = i - x * N;
+            const int x = i / RW, col = c0 + i - x * RW;
             const int g = x < S::NA ? bra.bfa + x : bra.bfb + (x - S::NA);
             if (v != 0.0) atomicAdd(&A.G[(size_t)g * N + col], v);
         }
@@ -718,10 +729,10 @@ __global__ void __launch_bounds__(256) eri_tensor_class_kernel(ClassArgs A, int 
 }
 
 template <int LA, int LB, int LC, int LD, typename T>
-inline size_t fock_class_lds_bytes(int N, int maxpp, int kj)
+inline size_t fock_class_lds_bytes(int RW, int maxpp, int kj)
 {
     using S = ClassShape<LA, LB, LC, LD>;
-    return sizeof(double) * ((size_t)(S::NA + S::NB) * N + S::IBUF_DOUBLES + (kj ? S::KJ_DOUBLES : 0))
+    return sizeof(double) * ((size_t)(S::NA + S::NB) * RW + S::IBUF_DOUBLES + (kj ? S::KJ_DOUBLES : 0))
          + sizeof(T) * (BOYS_TAYLOR * BOYS_NT + (size_t)(maxpp < PSLICE ? maxpp : PSLICE) * S::BPS)
          + sizeof(int) * (ClassList<S::QB>::QLIST + 4 + S::QB + 1 + CLASS_BLOCK / 64);
 }

@@ -57,11 +57,14 @@ struct BFDesc {
     double x, y, z;
     int32_t l, m, n;
     int32_t p0, np;
-    int32_t pad;
+    int32_t pad;   // molecule of the function (0 for a single molecule)
 };
 
+// plist (optional): the (p, q) pairs to compute, one molecule's pairs after
+// another, for a basis that holds several molecules block-diagonally; the
+// other elements of the output are cleared.
 hipError_t launch_eri_diag(int N, const BFDesc* bfs, const double* pexp, const double* pw, const double* boys,
-                           double* out, hipStream_t st);
+                           double* out, hipStream_t st, const int2* plist = nullptr, int64_t npl = 0);
 hipError_t launch_eri_bf_list(int64_t nq, const int64_t* ijkl, const BFDesc* bfs, const double* pexp,
                               const double* pw, const double* boys, double* out, hipStream_t st);
 hipError_t launch_canonical_quartets(int mode, int N, const BFDesc* bfs, const double* pexp, const double* pw,
@@ -70,7 +73,8 @@ hipError_t launch_canonical_quartets(int mode, int N, const BFDesc* bfs, const d
 hipError_t launch_one_electron(int nbf, const BFDesc* bfs, const double* pexp, const double* pw, int natom,
                                const double* atoms /* natom x 4: Z, x, y, z */, double cx, double cy, double cz,
                                const double* boys, double* S, double* T, double* V, double* M, double* Lm,
-                               hipStream_t st);
+                               hipStream_t st, const int2* plist = nullptr, int64_t npl = 0,
+                               const int* atom_mol = nullptr);
 // Survivor list of one class; with surv2, the quartets whose bound lies in
 // [tol, split) go to the second list instead (the FP32 tier).
 hipError_t launch_screen_class(int boff, int nb, int koff, int nk, int tri, const ShellPair* pairs,

@@ -13,6 +13,8 @@ struct ShellPair {
     int32_t bfa, bfb;    // first basis-function index of each shell
     int32_t pp0, npp;    // range in the primitive-pair table
     int32_t same;        // sa == sb
+    int32_t mol;         // molecule of the pair (a basis can hold several, block-diagonally)
+    int32_t c0;          // first basis function of that molecule
     int32_t pad;
     double q;            // max over the pair's functions of sqrt((ab|ab))
 };
@@ -55,7 +57,8 @@ struct RowArgs {
     int koff, nk;               // ket pairs of the class
     int tri;                    // same pair class: ket index <= bra index
     int seg;                    // ket pairs per work unit
-    const int4* units;          // work units: {bra pair, first and end bra primitive pair, first ket}
+    int rw;                     // width of the LDS rows: the largest molecule of the basis
+    const int4* units;          // work units: {bra pair, first | end << 16 bra primitive pair, first ket, end ket}
     int nunits;
     int maxpp;                  // largest primitive-pair count of the class's bra pairs
     int kj;                     // LDS holds the ket-side Coulomb buffer (classes that have one)

@@ -101,16 +101,24 @@ __device__ double eri_bf_generic(const BFDesc& a, const BFDesc& b, const BFDesc&
 __global__ void __launch_bounds__(256) eri_diag_wave_kernel(int N, const BFDesc* __restrict__ bfs,
                                                             const double* __restrict__ pexp,
                                                             const double* __restrict__ pw,
-                                                            const double* __restrict__ boys, double* out)
+                                                            const double* __restrict__ boys, double* out,
+                                                            const int2* __restrict__ plist, int64_t npl)
 {
     const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
-    const int64_t npair = (int64_t)N * (N + 1) / 2;
+    const int64_t npair = plist ? npl : (int64_t)N * (N + 1) / 2;
     if (w >= npair) return;   // uniform across the wavefront
-    int64_t r = (int64_t)((sqrt(8.0 * (double)w + 1.0) - 1.0) * 0.5);
-    while (r * (r + 1) / 2 > w) --r;
-    while ((r + 1) * (r + 2) / 2 <= w) ++r;
-    const int p = (int)r, q = (int)(w - r * (r + 1) / 2);
+    int p, q;
+    if (plist) {   // a block-diagonal basis: the pairs of each molecule
+        p = plist[w].x;
+        q = plist[w].y;
+    } else {
+        int64_t r = (int64_t)((sqrt(8.0 * (double)w + 1.0) - 1.0) * 0.5);
+        while (r * (r + 1) / 2 > w) --r;
+        while ((r + 1) * (r + 2) / 2 <= w) ++r;
+        p = (int)r;
+        q = (int)(w - r * (r + 1) / 2);
+    }
     const BFDesc a = bfs[p], b = bfs[q];
     const int nab = a.np * b.np;
     double total = 0.0;
@@ -129,12 +137,17 @@ __global__ void __launch_bounds__(256) eri_diag_wave_kernel(int N, const BFDesc*
 }
 
 hipError_t launch_eri_diag(int N, const BFDesc* bfs, const double* pexp, const double* pw, const double* boys,
-                           double* out, hipStream_t st)
+                           double* out, hipStream_t st, const int2* plist, int64_t npl)
 {
-    const int64_t npair = (int64_t)N * (N + 1) / 2;
+    const int64_t npair = plist ? npl : (int64_t)N * (N + 1) / 2;
     if (npair <= 0) return hipSuccess;
+    if (plist) {   // only the molecules' own pairs are computed: clear the rest
+        const hipError_t e = launch_fill((size_t)N * N, out, 0.0, st);
+        if (e != hipSuccess) return e;
+    }
     const int64_t grid = (npair + 3) / 4;   // four wavefronts per 256-thread block
-    hipLaunchKernelGGL(eri_diag_wave_kernel, dim3((unsigned)grid), dim3(256), 0, st, N, bfs, pexp, pw, boys, out);
+    hipLaunchKernelGGL(eri_diag_wave_kernel, dim3((unsigned)grid), dim3(256), 0, st, N, bfs, pexp, pw, boys, out,
+                       plist, npl);
     return hipGetLastError();
 }
 
@@ -248,16 +261,24 @@ __global__ void __launch_bounds__(256) one_electron_kernel(int nbf, const BFDesc
                                                            const double* __restrict__ pw, int natom,
                                                            const double* __restrict__ atoms, double cx, double cy,
                                                            double cz, const double* __restrict__ boys, double* S,
-                                                           double* T, double* V, double* M, double* Lm)
+                                                           double* T, double* V, double* M, double* Lm,
+                                                           const int2* __restrict__ plist, int64_t npl,
+                                                           const int* __restrict__ atom_mol)
 {
     const int64_t idx = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
-    const int64_t npair = (int64_t)nbf * (nbf + 1) / 2;
+    const int64_t npair = plist ? npl : (int64_t)nbf * (nbf + 1) / 2;
     if (idx >= npair) return;   // uniform across the wavefront
-    int i = (int)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
-    while ((int64_t)i * (i + 1) / 2 > idx) --i;
-    while ((int64_t)(i + 1) * (i + 2) / 2 <= idx) ++i;
-    const int j = (int)(idx - (int64_t)i * (i + 1) / 2);
+    int i, j;
+    if (plist) {   // a block-diagonal basis: the pairs of each molecule
+        i = plist[idx].x;
+        j = plist[idx].y;
+    } else {
+        i = (int)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
+        while ((int64_t)i * (i + 1) / 2 > idx) --i;
+        while ((int64_t)(i + 1) * (i + 2) / 2 <= idx) ++i;
+        j = (int)(idx - (int64_t)i * (i + 1) / 2);
+    }
     const BFDesc a = bfs[i], b = bfs[j];
     const double A[3] = {a.x, a.y, a.z}, B[3] = {b.x, b.y, b.z}, Cc[3] = {cx, cy, cz};
     const int l1[3] = {a.l, a.m, a.n}, l2[3] = {b.l, b.m, b.n};
@@ -308,8 +329,9 @@ __global__ void __launch_bounds__(256) one_electron_kernel(int nbf, const BFDesc
             lam[1] += -S0[1] * (S1[2] * D1[0] - S1[0] * D1[2]) * pre;
             lam[2] += -S0[2] * (S1[0] * D1[1] - S1[1] * D1[0]) * pre;
         }
-        // nuclear attraction of nucleus `at` (cython/onee.pyx:175-192)
-        if (at < natom) {
+        // nuclear attraction of nucleus `at` (cython/onee.pyx:175-192), the
+        // nuclei of the pair's own molecule
+        if (at < natom && !(atom_mol && atom_mol[at] != a.pad)) {
             const int Lv = l1[0] + l2[0] + l1[1] + l2[1] + l1[2] + l2[2];
             const double Z = atoms[4 * at];
             const double X = P[0] - atoms[4 * at + 1], Y = P[1] - atoms[4 * at + 2], W = P[2] - atoms[4 * at + 3];
@@ -352,13 +374,25 @@ __global__ void __launch_bounds__(256) one_electron_kernel(int nbf, const BFDesc
 
 hipError_t launch_one_electron(int nbf, const BFDesc* bfs, const double* pexp, const double* pw, int natom,
                                const double* atoms, double cx, double cy, double cz, const double* boys, double* S,
-                               double* T, double* V, double* M, double* Lm, hipStream_t st)
+                               double* T, double* V, double* M, double* Lm, hipStream_t st, const int2* plist,
+                               int64_t npl, const int* atom_mol)
 {
-    const int64_t npair = (int64_t)nbf * (nbf + 1) / 2;
+    const int64_t npair = plist ? npl : (int64_t)nbf * (nbf + 1) / 2;
     if (npair <= 0) return hipSuccess;
+    if (plist) {   // only the molecules' own pairs are computed: clear the rest
+        const size_t nn = (size_t)nbf * nbf;
+        for (double* x : {S, T, V}) {
+            const hipError_t e = launch_fill(nn, x, 0.0, st);
+            if (e != hipSuccess) return e;
+        }
+        for (double* x : {M, Lm}) {
+            const hipError_t e = launch_fill(3 * nn, x, 0.0, st);
+            if (e != hipSuccess) return e;
+        }
+    }
     // four wavefronts (basis-function pairs) per 256-thread block
     hipLaunchKernelGGL(one_electron_kernel, dim3((unsigned)((npair + 3) / 4)), dim3(256), 0, st, nbf, bfs, pexp,
-                       pw, natom, atoms, cx, cy, cz, boys, S, T, V, M, Lm);
+                       pw, natom, atoms, cx, cy, cz, boys, S, T, V, M, Lm, plist, npl, atom_mol);
     return hipGetLastError();
 }
 
@@ -396,7 +430,7 @@ __global__ void __launch_bounds__(256) screen_class_kernel(int boff, int nb, int
                 Q = (int)(idx % nk);
             }
             if (keep_all) {
-                keep = true;
+                keep = pairs[boff + P].mol == pairs[koff + Q].mol;
             } else {
                 const ShellPair bra = pairs[boff + P], ket = pairs[koff + Q];
                 const size_t ns = (size_t)nshell;
@@ -404,7 +438,7 @@ __global__ void __launch_bounds__(256) screen_class_kernel(int boff, int nb, int
                                             fmax(Dsh[bra.sa * ns + ket.sa], Dsh[bra.sa * ns + ket.sb])),
                                        fmax(Dsh[bra.sb * ns + ket.sa], Dsh[bra.sb * ns + ket.sb]));
                 const double bound = bra.q * ket.q * dm;
-                keep = bound >= tol;
+                keep = bound >= tol && bra.mol == ket.mol;   // quartets of one molecule only
                 // with a second list: the quartets below `split` go there
                 low = keep && surv2 && bound < split;
             }

@@ -56,6 +56,11 @@ lib.mmd_one_electron.restype = _c_int
 lib.mmd_one_electron.argtypes = [_ctx_p, _vp, _vp, _vp, _vp, _vp, _vp]
 lib.mmd_schwarz.restype = _c_int
 lib.mmd_schwarz.argtypes = [_ctx_p, _vp]
+lib.mmd_set_molecules.restype = _c_int
+lib.mmd_set_molecules.argtypes = [_ctx_p, _c_int, _vp, _vp]
+lib.mmd_batch_rhf.restype = _c_int
+lib.mmd_batch_rhf.argtypes = [_ctx_p, _vp, _vp, _vp, _c_double, _c_double, _c_int, _c_int, _c_int, _vp, _vp, _vp,
+                              _vp, _vp]
 lib.mmd_overlap_powers.restype = _c_int
 lib.mmd_overlap_powers.argtypes = [_ctx_p, _vp, _vp, _vp, ctypes.POINTER(_c_int)]
 lib.mmd_dgemm.restype = _c_int
@@ -119,7 +124,8 @@ EXPORTED = [
     "mmd_set_basis", "mmd_set_atoms", "mmd_one_electron", "mmd_schwarz", "mmd_set_schwarz",
     "mmd_eri_tensor", "mmd_eri_list", "mmd_fock_delta", "mmd_fock_delta_noalloc", "mmd_jk_tensor", "mmd_boys",
     "mmd_last_fock_ms", "mmd_set_profiling", "mmd_fock_stats", "mmd_set_precision", "mmd_fock_tiers",
-    "mmd_set_streams", "mmd_set_lds_limit", "mmd_overlap_powers", "mmd_dgemm", "mmd_scf_begin", "mmd_scf_step", "mmd_scf_density", "mmd_scf_get", "mmd_ao2mo",
+    "mmd_set_streams", "mmd_set_lds_limit", "mmd_overlap_powers", "mmd_dgemm", "mmd_set_molecules",
+    "mmd_batch_rhf", "mmd_scf_begin", "mmd_scf_step", "mmd_scf_density", "mmd_scf_get", "mmd_ao2mo",
     "mmd_mlp_create", "mmd_mlp_destroy", "mmd_mlp_nparam", "mmd_mlp_set_params", "mmd_mlp_get_params",
     "mmd_mlp_forward", "mmd_mlp_loss_grad", "mmd_mlp_train",
 ]
@@ -207,6 +213,31 @@ class Context:
         M, L = np.empty((3, N, N)), np.empty((3, N, N))
         check(lib.mmd_one_electron(self._h, _ptr(c), _ptr(S), _ptr(T), _ptr(V), _ptr(M), _ptr(L)))
         return S, T, V, M, L
+
+    def set_molecules(self, nbf_per_mol, natom_per_mol):
+        """The basis and nuclei are those of several molecules, block-diagonally."""
+        nb = np.ascontiguousarray(nbf_per_mol, dtype=np.int32)
+        na = np.ascontiguousarray(natom_per_mol, dtype=np.int32)
+        check(lib.mmd_set_molecules(self._h, len(nb), _ptr(nb), _ptr(na)))
+        self.nmol = len(nb)
+        self.mol_nbf = nb
+
+    def batch_rhf(self, nocc, enuc, P0=None, conver=1e-8, acc2e=1e-12, maxiter=100, diis=True, use_graph=True,
+                  keep_density=False):
+        """Direct RHF of every molecule of the basis at once (mmd_batch_rhf);
+        returns (energy, iterations, converged[, packed densities])."""
+        M = self.nmol
+        no = np.ascontiguousarray(nocc, dtype=np.int32)
+        en = _f64(enuc)
+        E = np.empty(M)
+        it = np.empty(M, dtype=np.int32)
+        cv = np.empty(M, dtype=np.int32)
+        ptot = int(np.sum(self.mol_nbf.astype(np.int64) ** 2))
+        P = np.empty(ptot) if keep_density else None
+        p0 = None if P0 is None else _f64(P0)
+        check(lib.mmd_batch_rhf(self._h, _ptr(no), _ptr(en), _ptr(p0), float(conver), float(acc2e), int(maxiter),
+                                int(bool(diis)), int(bool(use_graph)), _ptr(E), _ptr(it), _ptr(cv), _ptr(P), None))
+        return (E, it, cv.astype(bool), P) if keep_density else (E, it, cv.astype(bool))
 
     def overlap_powers(self, S=None):
         """(X, U, iterations): S^-1/2 and S^1/2 on the GPU (Newton-Schulz on

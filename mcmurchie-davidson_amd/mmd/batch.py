@@ -234,3 +234,100 @@ if __name__ == "__main__" and "--worker" in sys.argv:
     _worker_main()
 elif __name__ == "__main__" and "--serve" in sys.argv:
     _serve_main(int(sys.argv[sys.argv.index("--serve") + 1]), int(sys.argv[sys.argv.index("--serve") + 2]))
+
+
+# ---------------------------------------------------------------------------
+# Batched RHF on the device (mmd_batch_rhf): groups of molecules in one
+# block-diagonal basis, one Fock build per iteration for the whole group and
+# the rest of the loop in one workgroup per molecule, replayed as a HIP graph.
+# ---------------------------------------------------------------------------
+_TEMPLATES = {}
+
+
+def _element_template(basis, Z):
+    """Basis functions of one atom of element Z at the origin: (lmn, nprim,
+    exps, weights) in the reference's order (mmd/molecule.py:57-88), with the
+    normalised weights norm * coef of cython/basis.pxi:87-120."""
+    key = (basis, Z)
+    t = _TEMPLATES.get(key)
+    if t is None:
+        from mmd.molecule import _SYMBOLS
+        mol = Molecule("\n0 1\n%s 0.0 0.0 0.0\n" % _SYMBOLS[Z], basis)
+        lmn = np.array([b.shell for b in mol.bfs], dtype=np.int32).reshape(-1, 3)
+        nprim = np.array([b.num_exps for b in mol.bfs], dtype=np.int32)
+        exps = np.concatenate([np.asarray(b.exps, dtype=np.float64) for b in mol.bfs])
+        w = np.concatenate([np.asarray(b.norm, dtype=np.float64) * np.asarray(b.coefs, dtype=np.float64)
+                            for b in mol.bfs])
+        t = _TEMPLATES[key] = (lmn, nprim, exps, w)
+    return t
+
+
+def _molecule_arrays(geometry, basis):
+    """(Z, xyz bohr, origin, lmn, nprim, exps, weights, nocc, E_nuc) of one
+    geometry string, as Molecule(geometry, basis) would build them."""
+    from mmd.molecule import BOHR_ANGSTROM, _SYMBOLS
+    rows = [ln.split() for ln in geometry.split("\n") if ln.strip()]
+    charge = int(rows[0][0])
+    Z = np.array([_SYMBOLS.index(r[0]) for r in rows[1:]], dtype=np.int64)
+    xyz = np.array([[float(x) for x in r[1:4]] for r in rows[1:]]) / BOHR_ANGSTROM
+    parts = [_element_template(basis, int(z)) for z in Z]
+    lmn = np.concatenate([p[0] for p in parts])
+    nprim = np.concatenate([p[1] for p in parts])
+    exps = np.concatenate([p[2] for p in parts])
+    w = np.concatenate([p[3] for p in parts])
+    origin = np.concatenate([np.repeat(xyz[i:i + 1], len(p[1]), axis=0) for i, p in enumerate(parts)])
+    d = np.linalg.norm(xyz[:, None, :] - xyz[None, :, :], axis=-1)
+    iu = np.triu_indices(len(Z), 1)
+    enuc = float(np.sum(np.outer(Z, Z)[iu] / d[iu]))
+    nocc = (int(Z.sum()) - charge) // 2
+    return Z, xyz, origin, lmn, nprim, exps, w, nocc, enuc
+
+
+def run_batch_device(geometries, basis="6-31g", group=64, guess="core", fp32_below=0.0, conver=1e-8,
+                     acc2e=1e-12, device=0, use_graph=True, keep_density=False, guess_model=None, ctx=None):
+    """Direct RHF energies of all geometries, `group` molecules at a time in
+    one block-diagonal basis on the GPU (mmd_batch_rhf).  guess: 'core' (the
+    reference's), 'sad' or 'learned' (the initial densities are formed per
+    molecule on the host side first).  fp32_below: the FP32 tier of the Fock
+    builds.  Returns one dict per geometry: energy, iterations, converged,
+    nbasis (and the density P with keep_density)."""
+    if group < 1:
+        raise ValueError("group must be >= 1")
+    ctx = ctx if ctx is not None else _native.Context(device)
+    out = [None] * len(geometries)
+    for g0 in range(0, len(geometries), group):
+        idx = list(range(g0, min(len(geometries), g0 + group)))
+        arrs = [_molecule_arrays(geometries[i], basis) for i in idx]
+        ctx.set_basis(np.concatenate([a[2] for a in arrs]), np.concatenate([a[3] for a in arrs]),
+                      np.concatenate([a[4] for a in arrs]), np.concatenate([a[5] for a in arrs]),
+                      np.concatenate([a[6] for a in arrs]))
+        ctx.set_atoms(np.concatenate([a[0] for a in arrs]).astype(np.float64),
+                      np.concatenate([a[1] for a in arrs]))
+        ctx.set_molecules([len(a[4]) for a in arrs], [len(a[0]) for a in arrs])
+        ctx.set_precision(fp32_below)
+        P0 = None
+        if guess != "core":
+            blocks = []
+            for i in idx:
+                mol = Molecule(geometries[i], basis)
+                if guess == "sad":
+                    from mmd.guess import sad_density
+                    blocks.append(np.real(sad_density(mol)).ravel())
+                elif guess == "learned":
+                    from mmd.learned import learned_density
+                    blocks.append(np.real(learned_density(mol, guess_model)).ravel())
+                else:
+                    raise ValueError("guess must be 'core', 'sad' or 'learned'")
+            P0 = np.concatenate(blocks)
+        res = ctx.batch_rhf([a[7] for a in arrs], [a[8] for a in arrs], P0=P0, conver=conver, acc2e=acc2e,
+                            use_graph=use_graph, keep_density=keep_density)
+        o = 0
+        for k, i in enumerate(idx):
+            n = len(arrs[k][4])
+            r = {"energy": float(res[0][k]), "iterations": int(res[1][k]), "converged": bool(res[2][k]),
+                 "nbasis": n}
+            if keep_density:
+                r["P"] = res[3][o:o + n * n].reshape(n, n).copy()
+            o += n * n
+            out[i] = r
+    return out

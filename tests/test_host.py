@@ -124,3 +124,24 @@ def test_screen_dict_matches_reference_layout():
     for p in range(5):
         for q in range(p + 1):
             assert d[p * (p + 1) // 2 + q] == (A + A.T)[p, q]
+
+
+def test_batch_molecule_arrays_match_molecule(oracle, basis_dir):
+    """The batch driver's per-molecule arrays (element templates) are the
+    basis and nuclear repulsion that Molecule and the oracle build."""
+    import os
+    import numpy as np
+    from geometries import water_cluster
+    from mmd.batch import _molecule_arrays
+    from mmd.molecule import Molecule
+    geom = water_cluster(3, 6.2, seed=4, perturb=True)
+    Z, xyz, origin, lmn, nprim, exps, w, nocc, enuc = _molecule_arrays(geom, "6-31g")
+    mol = Molecule(geom, "6-31g")
+    assert np.array_equal(origin, np.array([b.origin for b in mol.bfs]))
+    assert np.array_equal(lmn, np.array([b.shell for b in mol.bfs]))
+    assert np.array_equal(nprim, np.array([b.num_exps for b in mol.bfs]))
+    assert np.array_equal(exps, np.concatenate([b.exps for b in mol.bfs]))
+    assert np.allclose(w, np.concatenate([b.norm * b.coefs for b in mol.bfs]), rtol=1e-15, atol=0)
+    assert nocc == mol.nocc
+    om = oracle.OracleMolecule(geom, os.path.join(basis_dir, "6-31g.gbs"))
+    assert abs(enuc - om.nuc_energy) < 1e-12

@@ -25,6 +25,8 @@ struct BatchState {
 
 struct BatchArgs {
     int nmol, N;
+    int maxn;               // largest molecule
+    size_t lds_bytes;       // LDS a workgroup may allocate on the device
     const BatchMol* mol;
     BatchState* state;
     double *S, *H, *X, *U, *F, *P, *Pold, *dP, *G;   // N x N, block-diagonal

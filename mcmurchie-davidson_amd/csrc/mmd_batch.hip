@@ -160,6 +160,16 @@ __device__ void wg_jacobi(int n, double* A, int lda, double* V, int ldv, double*
     __syncthreads();
 }
 
+// LDS vectors of a molecule workgroup (the matrices are dynamic LDS).
+struct WgSmall {
+    double w[BATCH_MAX_NBF];
+    double cs[BATCH_MAX_NBF + 2];
+    int perm[BATCH_MAX_NBF + 2];
+    double red[BT / 64];
+    double Bm[9 * 9];
+    double rhs[9];
+};
+
 // LDS of a molecule workgroup: one n x n matrix (leading dimension n) plus
 // small vectors; n <= BATCH_MAX_NBF.
 struct WgLds {
@@ -349,6 +359,186 @@ __global__ void __launch_bounds__(BT) batch_step_kernel(BatchArgs B, int build, 
     for (int k = threadIdx.x; k < n; k += blockDim.x) B.mo[M.f0 + k] = L.w[k];
 }
 
+// C (LDS, n x n) = op(A) op(B) with A, B n x n in LDS (leading dimension n).
+__device__ void lds_gemm(int n, bool tA, const double* A, bool tB, const double* B, double* C)
+{
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+        const int i = e / n, j = e - i * n;
+        double s = 0.0;
+        if (!tA && !tB) {
+            const double* a = A + i * n;
+            for (int k = 0; k < n; ++k) s = fma(a[k], B[k * n + j], s);
+        } else {
+            for (int k = 0; k < n; ++k)
+                s = fma(tA ? A[k * n + i] : A[i * n + k], tB ? B[j * n + k] : B[k * n + j], s);
+        }
+        C[e] = s;
+    }
+    __syncthreads();
+}
+
+// load the n x n block at M (leading dimension N) into LDS
+__device__ void lds_load(int n, const double* M, int N, double* L)
+{
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) L[e] = M[(size_t)(e / n) * N + e % n];
+    __syncthreads();
+}
+
+// The step of batch_step_kernel with its matrices in LDS: three n x n
+// matrices (dynamic shared memory) hold P S, F P S, the error, X, F_diis, FO
+// and the eigenvectors in turn, so the products and the Jacobi rotations run
+// from LDS.  Same operations in the same order as batch_step_kernel.
+__global__ void __launch_bounds__(BT) batch_step_lds_kernel(BatchArgs B, int build, int diis, double conver)
+{
+    extern __shared__ double dyn[];
+    __shared__ WgSmall L;
+    const int mi = blockIdx.x;
+    const BatchMol M = B.mol[mi];
+    BatchState& Sx = B.state[mi];
+    const int n = M.n, N = B.N, f0 = M.f0;
+    if (Sx.converged) return;
+    const size_t nn = (size_t)n * n;
+    double* M0 = dyn;
+    double* M1 = dyn + nn;
+    double* M2 = dyn + 2 * nn;
+    double* F = blk(B.F, N, f0);
+    double* P = blk(B.P, N, f0);
+    double* Po = blk(B.Pold, N, f0);
+    double* dP = blk(B.dP, N, f0);
+    const double* G = blk(B.G, N, f0);
+    const double* H = blk(B.H, N, f0);
+    const double* S = blk(B.S, N, f0);
+    const double* X = blk(B.X, N, f0);
+    double* Eset = B.diis + M.d8;
+    double* Fset = Eset + (size_t)8 * nn;
+    bool fo_from_lds = false;   // F_diis in M1
+    if (build) {
+        for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+            const size_t o = (size_t)(e / n) * N + e % n;
+            F[o] += G[o];
+            Po[o] = P[o];
+        }
+        __syncthreads();
+        if (diis) {
+            const int cnt = Sx.ndiis;
+            const int slot = cnt == 8 ? Sx.order[0] : cnt;
+            __syncthreads();
+            if (cnt == 8 && threadIdx.x == 0) {
+                for (int i = 0; i < 7; ++i) Sx.order[i] = Sx.order[i + 1];
+                for (int i = 0; i < 7; ++i)
+                    for (int j = 0; j < 7; ++j) Sx.Bm[i * 8 + j] = Sx.Bm[(i + 1) * 8 + (j + 1)];
+            }
+            __syncthreads();
+            const int nw = cnt == 8 ? 8 : cnt + 1;
+            double* Es = Eset + (size_t)slot * nn;
+            lds_load(n, P, N, M0);
+            lds_load(n, S, N, M1);
+            lds_gemm(n, false, M0, false, M1, M2);                 // P S
+            lds_load(n, F, N, M0);
+            lds_gemm(n, false, M0, false, M2, M1);                 // F P S
+            for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+                const int i = e / n, j = e - i * n;
+                M2[e] = M1[i * n + j] - M1[j * n + i];             // FPS - SPF
+            }
+            __syncthreads();
+            lds_load(n, X, N, M0);
+            lds_gemm(n, false, M0, false, M2, M1);                 // X (FPS - SPF)
+            lds_gemm(n, false, M1, false, M0, M2);                 // ... X: the error
+            for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+                Es[e] = M2[e];
+                Fset[(size_t)slot * nn + e] = F[(size_t)(e / n) * N + e % n];
+            }
+            if (threadIdx.x == 0) Sx.order[nw - 1] = slot;
+            __syncthreads();
+            for (int k = 0; k < nw; ++k) {
+                const int sk = Sx.order[k];
+                double d = 0.0;
+                if (sk == slot) {
+                    for (int e = threadIdx.x; e < n * n; e += blockDim.x) d = fma(M2[e], M2[e], d);
+                } else {
+                    const double* Ek = Eset + (size_t)sk * nn;
+                    for (int e = threadIdx.x; e < n * n; e += blockDim.x) d = fma(Ek[e], M2[e], d);
+                }
+                d = wg_sum(d, L.red);
+                if (threadIdx.x == 0) Sx.Bm[(nw - 1) * 8 + k] = Sx.Bm[k * 8 + (nw - 1)] = d;
+            }
+            if (threadIdx.x == 0) {
+                Sx.ndiis = nw;
+                const int mm = nw + 1;
+                double* a = L.Bm;
+                double* r = L.rhs;
+                for (int i = 0; i < mm; ++i) {
+                    for (int j = 0; j < mm; ++j)
+                        a[i * 9 + j] = (i < nw && j < nw) ? Sx.Bm[i * 8 + j] : ((i == nw && j == nw) ? 0.0 : -1.0);
+                    r[i] = i == nw ? -1.0 : 0.0;
+                }
+                bool ok = true;
+                for (int col = 0; col < mm; ++col) {
+                    int piv = col;
+                    for (int i = col + 1; i < mm; ++i)
+                        if (fabs(a[i * 9 + col]) > fabs(a[piv * 9 + col])) piv = i;
+                    if (a[piv * 9 + col] == 0.0) { ok = false; break; }
+                    if (piv != col) {
+                        for (int j = 0; j < mm; ++j) { const double t = a[col * 9 + j]; a[col * 9 + j] = a[piv * 9 + j]; a[piv * 9 + j] = t; }
+                        const double t = r[col]; r[col] = r[piv]; r[piv] = t;
+                    }
+                    for (int i = col + 1; i < mm; ++i) {
+                        const double f = a[i * 9 + col] / a[col * 9 + col];
+                        for (int j = col; j < mm; ++j) a[i * 9 + j] -= f * a[col * 9 + j];
+                        r[i] -= f * r[col];
+                    }
+                }
+                for (int i = mm - 1; i >= 0 && ok; --i) {
+                    double v = r[i];
+                    for (int j = i + 1; j < mm; ++j) v -= a[i * 9 + j] * r[j];
+                    r[i] = v / a[i * 9 + i];
+                }
+                if (!ok) B.flag[mi] = -2;
+            }
+            __syncthreads();
+            for (int e = threadIdx.x; e < n * n; e += blockDim.x) {   // F_diis into M1
+                double v = 0.0;
+                for (int k = 0; k < nw; ++k) v = fma(L.rhs[k], Fset[(size_t)Sx.order[k] * nn + e], v);
+                M1[e] = v;
+            }
+            __syncthreads();
+            fo_from_lds = true;
+        }
+    }
+    // FO = X^T F X  (X in M0, F in M1)
+    lds_load(n, X, N, M0);
+    if (!fo_from_lds) lds_load(n, F, N, M1);
+    lds_gemm(n, true, M0, false, M1, M2);
+    lds_gemm(n, false, M2, false, M0, M1);                         // FO in M1
+    wg_jacobi(n, M1, n, M2, n, L.w, L.perm, L.cs, L.red);          // C' in M2 (LDS)
+    lds_gemm(n, false, M0, false, M2, M1);                         // C = X C' in M1
+    const int nocc = M.nocc;
+    double e_el = 0.0, rms = 0.0;
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+        const int i = e / n, j = e - i * n;
+        double v = 0.0;
+        for (int k = 0; k < nocc; ++k) v = fma(M1[i * n + k], M1[j * n + k], v);
+        const size_t o = (size_t)i * N + j;
+        P[o] = v;
+        const double d = v - Po[o];
+        dP[o] = d;
+        rms = fma(d, d, rms);
+        e_el = fma(H[o] + F[o], v, e_el);
+    }
+    e_el = wg_sum(e_el, L.red);
+    rms = wg_sum(rms, L.red);
+    if (threadIdx.x == 0) {
+        const double energy = e_el + M.enuc;
+        const int step = Sx.iterations;
+        Sx.delta_energy = step > 0 ? energy - Sx.energy : 1e20;
+        Sx.energy = energy;
+        Sx.prms = step > 0 ? sqrt(rms) : 1e20;
+        if (step > 0 && Sx.prms < conver) Sx.converged = 1;
+        else Sx.iterations = step + 1;
+    }
+    for (int k = threadIdx.x; k < n; k += blockDim.x) B.mo[M.f0 + k] = L.w[k];
+}
+
 // dP of converged molecules: zero, so the next Fock build skips them.
 __global__ void batch_clear_converged_kernel(BatchArgs B)
 {
@@ -385,7 +575,14 @@ hipError_t launch_batch_overlap(const BatchArgs& B, hipStream_t st)
 
 hipError_t launch_batch_step(const BatchArgs& B, int build, int diis, double conver, hipStream_t st)
 {
-    hipLaunchKernelGGL(batch_step_kernel, dim3(B.nmol), dim3(BT), 0, st, B, build, diis, conver);
+    // the all-LDS step where three n x n matrices of the largest molecule fit
+    // beside the small vectors; the global-memory step otherwise
+    if (B.lds_bytes >= 3 * (size_t)B.maxn * B.maxn * sizeof(double) + sizeof(WgSmall)) {
+        const size_t dyn = 3 * (size_t)B.maxn * B.maxn * sizeof(double);
+        hipLaunchKernelGGL(batch_step_lds_kernel, dim3(B.nmol), dim3(BT), dyn, st, B, build, diis, conver);
+    } else {
+        hipLaunchKernelGGL(batch_step_kernel, dim3(B.nmol), dim3(BT), 0, st, B, build, diis, conver);
+    }
     if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
     hipLaunchKernelGGL(batch_clear_converged_kernel, dim3(B.nmol), dim3(BT), 0, st, B);
     return hipGetLastError();

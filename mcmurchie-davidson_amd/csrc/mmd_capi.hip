@@ -178,8 +178,8 @@ struct mmd_ctx {
     // rows); plist: the same-molecule function pairs p >= q.
     int nmol = 1;
     std::vector<int> mol_f0{0}, mol_n{0}, mol_s0{0, 0};
-    std::vector<int> atom_mol;
-    DevBuf<int> d_atom_mol;
+    std::vector<int2> mol_atoms;        // per molecule: first atom, number of atoms
+    DevBuf<int2> d_mol_atoms;
     DevBuf<int2> d_plist;
     int64_t npl = 0;
     int rw = 0;
@@ -208,7 +208,8 @@ struct mmd_ctx {
     int unit_seg[NQCLASS] = {0};
     bool units_ok = false;
     size_t max_lds = 65536;             // LDS a class launch may use (mmd_set_lds_limit)
-    size_t max_lds_device = 65536;      // the device's limit per workgroup
+    size_t max_lds_device = 65536;      // the device's limit per workgroup (class kernels use up to 64 KiB)
+    size_t lds_per_block = 65536;       // the device's LDS limit per workgroup, as reported
     bool last_row_kernel[NQCLASS] = {};
     int64_t st_nquart[NQCLASS] = {0};
 
@@ -847,6 +848,7 @@ int mmd_ctx_create(mmd_ctx** out, int device_id)
         lds_max > 0)
         c->max_lds = std::min<size_t>((size_t)lds_max, 65536);
     c->max_lds_device = c->max_lds;
+    if (lds_max > 0) c->lds_per_block = (size_t)lds_max;
     hipError_t e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
@@ -935,7 +937,7 @@ int mmd_set_basis(mmd_ctx* ctx, int nbf, const double* origin, const int32_t* lm
     ctx->nmol = 1;
     ctx->mol_f0.assign(1, 0);
     ctx->mol_n.assign(1, nbf);
-    ctx->atom_mol.clear();
+    ctx->mol_atoms.clear();
     ctx->npl = 0;
     ctx->rw = nbf;
     HIP_TRY(ctx->d_bf.reserve(nbf));
@@ -1014,17 +1016,15 @@ int mmd_set_molecules(mmd_ctx* ctx, int nmol, const int32_t* nbf_per_mol, const 
         }
         f += nbf_per_mol[m];
     }
-    ctx->atom_mol.clear();
-    for (int m = 0; m < nmol; ++m)
-        for (int a = 0; a < natom_per_mol[m]; ++a) ctx->atom_mol.push_back(m);
+    ctx->mol_atoms.clear();
+    for (int m = 0, a0 = 0; m < nmol; a0 += natom_per_mol[m], ++m) ctx->mol_atoms.push_back(make_int2(a0, natom_per_mol[m]));
     ctx->npl = (int64_t)pl.size();
     HIP_TRY(hipMemcpyAsync(ctx->d_bf.p, ctx->bf.data(), sizeof(BFDesc) * ctx->nbf, hipMemcpyHostToDevice, ctx->st));
     HIP_TRY(ctx->d_plist.reserve(pl.size()));
     HIP_TRY(hipMemcpyAsync(ctx->d_plist.p, pl.data(), sizeof(int2) * pl.size(), hipMemcpyHostToDevice, ctx->st));
-    HIP_TRY(ctx->d_atom_mol.reserve(std::max<size_t>(1, ctx->atom_mol.size())));
-    if (!ctx->atom_mol.empty())
-        HIP_TRY(hipMemcpyAsync(ctx->d_atom_mol.p, ctx->atom_mol.data(), sizeof(int) * ctx->atom_mol.size(),
-                               hipMemcpyHostToDevice, ctx->st));
+    HIP_TRY(ctx->d_mol_atoms.reserve(ctx->mol_atoms.size()));
+    HIP_TRY(hipMemcpyAsync(ctx->d_mol_atoms.p, ctx->mol_atoms.data(), sizeof(int2) * ctx->mol_atoms.size(),
+                           hipMemcpyHostToDevice, ctx->st));
     HIP_TRY(hipStreamSynchronize(ctx->st));
     build_shells(ctx);
     if (ctx->shells_ok) {
@@ -1050,7 +1050,7 @@ int mmd_one_electron(mmd_ctx* ctx, const double center[3], double* S, double* T,
     HIP_TRY(launch_one_electron(ctx->nbf, ctx->d_bf.p, ctx->d_bf_exp.p, ctx->d_bf_w.p, ctx->natom,
                                 ctx->natom ? ctx->d_atoms.p : nullptr, center[0], center[1], center[2], ctx->d_boys.p,
                                 d.p, d.p + nn, d.p + 2 * nn, d.p + 3 * nn, d.p + 6 * nn, ctx->st,
-                                several ? ctx->d_plist.p : nullptr, ctx->npl, several ? ctx->d_atom_mol.p : nullptr));
+                                several ? ctx->d_plist.p : nullptr, ctx->npl, several ? ctx->d_mol_atoms.p : nullptr));
     HIP_TRY(hipMemcpyAsync(S, d.p, sizeof(double) * nn, hipMemcpyDeviceToHost, ctx->st));
     HIP_TRY(hipMemcpyAsync(T, d.p + nn, sizeof(double) * nn, hipMemcpyDeviceToHost, ctx->st));
     HIP_TRY(hipMemcpyAsync(V, d.p + 2 * nn, sizeof(double) * nn, hipMemcpyDeviceToHost, ctx->st));
@@ -1584,7 +1584,7 @@ int mmd_batch_rhf(mmd_ctx* ctx, const int32_t* nocc, const double* enuc, const d
     HIP_TRY(launch_one_electron(N, c->d_bf.p, c->d_bf_exp.p, c->d_bf_w.p, c->natom,
                                 c->natom ? c->d_atoms.p : nullptr, 0.0, 0.0, 0.0, c->d_boys.p, S, S + nn, S + 2 * nn,
                                 S + 3 * nn, S + 6 * nn, st, M > 1 ? c->d_plist.p : nullptr, c->npl,
-                                M > 1 ? c->d_atom_mol.p : nullptr));
+                                M > 1 ? c->d_mol_atoms.p : nullptr));
     for (DevBuf<double>* b : {&c->b_H, &c->b_X, &c->b_U, &c->b_F, &c->b_P, &c->b_Pold, &c->b_dP, &c->b_G})
         HIP_TRY(b->reserve(nn));
     HIP_TRY(c->b_work.reserve(std::max<size_t>(1, wtot)));
@@ -1636,6 +1636,8 @@ int mmd_batch_rhf(mmd_ctx* ctx, const int32_t* nocc, const double* enuc, const d
     BatchArgs B;
     B.nmol = M;
     B.N = N;
+    B.maxn = c->rw;
+    B.lds_bytes = c->lds_per_block;
     B.mol = c->b_mol.p;
     B.state = c->b_state.p;
     B.S = S;

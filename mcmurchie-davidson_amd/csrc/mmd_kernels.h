@@ -74,7 +74,7 @@ hipError_t launch_one_electron(int nbf, const BFDesc* bfs, const double* pexp, c
                                const double* atoms /* natom x 4: Z, x, y, z */, double cx, double cy, double cz,
                                const double* boys, double* S, double* T, double* V, double* M, double* Lm,
                                hipStream_t st, const int2* plist = nullptr, int64_t npl = 0,
-                               const int* atom_mol = nullptr);
+                               const int2* mol_atoms = nullptr /* per molecule: first atom, count */);
 // Survivor list of one class; with surv2, the quartets whose bound lies in
 // [tol, split) go to the second list instead (the FP32 tier).
 hipError_t launch_screen_class(int boff, int nb, int koff, int nk, int tri, const ShellPair* pairs,

@@ -263,7 +263,7 @@ __global__ void __launch_bounds__(256) one_electron_kernel(int nbf, const BFDesc
                                                            double cz, const double* __restrict__ boys, double* S,
                                                            double* T, double* V, double* M, double* Lm,
                                                            const int2* __restrict__ plist, int64_t npl,
-                                                           const int* __restrict__ atom_mol)
+                                                           const int2* __restrict__ mol_atoms)
 {
     const int64_t idx = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
@@ -283,7 +283,14 @@ __global__ void __launch_bounds__(256) one_electron_kernel(int nbf, const BFDesc
     const double A[3] = {a.x, a.y, a.z}, B[3] = {b.x, b.y, b.z}, Cc[3] = {cx, cy, cz};
     const int l1[3] = {a.l, a.m, a.n}, l2[3] = {b.l, b.m, b.n};
     double s = 0, t = 0, v = 0, mu[3] = {0, 0, 0}, lam[3] = {0, 0, 0};
-    const int nat = natom > 0 ? natom : 1;
+    // nuclei of the pair's own molecule: a contiguous range of the atom list
+    int at0 = 0, nat_mol = natom;
+    if (mol_atoms) {
+        const int2 r = mol_atoms[a.pad];
+        at0 = r.x;
+        nat_mol = r.y;
+    }
+    const int nat = nat_mol > 0 ? nat_mol : 1;
     const int nwork = a.np * b.np * nat;
     for (int k = lane; k < nwork; k += 64) {
         const int at = k % nat, ipp = k / nat;
@@ -329,12 +336,13 @@ __global__ void __launch_bounds__(256) one_electron_kernel(int nbf, const BFDesc
             lam[1] += -S0[1] * (S1[2] * D1[0] - S1[0] * D1[2]) * pre;
             lam[2] += -S0[2] * (S1[0] * D1[1] - S1[1] * D1[0]) * pre;
         }
-        // nuclear attraction of nucleus `at` (cython/onee.pyx:175-192), the
-        // nuclei of the pair's own molecule
-        if (at < natom && !(atom_mol && atom_mol[at] != a.pad)) {
+        // nuclear attraction of nucleus `at` of the pair's molecule
+        // (cython/onee.pyx:175-192)
+        if (at < nat_mol) {
             const int Lv = l1[0] + l2[0] + l1[1] + l2[1] + l1[2] + l2[2];
-            const double Z = atoms[4 * at];
-            const double X = P[0] - atoms[4 * at + 1], Y = P[1] - atoms[4 * at + 2], W = P[2] - atoms[4 * at + 3];
+            const double* nuc = atoms + 4 * (at0 + at);
+            const double Z = nuc[0];
+            const double X = P[0] - nuc[1], Y = P[1] - nuc[2], W = P[2] - nuc[3];
             double F[2 * MAX_L + 1];
             boys_runtime(Lv, p * (X * X + Y * Y + W * W), boys, F);
             double sc = 1.0;
@@ -375,7 +383,7 @@ __global__ void __launch_bounds__(256) one_electron_kernel(int nbf, const BFDesc
 hipError_t launch_one_electron(int nbf, const BFDesc* bfs, const double* pexp, const double* pw, int natom,
                                const double* atoms, double cx, double cy, double cz, const double* boys, double* S,
                                double* T, double* V, double* M, double* Lm, hipStream_t st, const int2* plist,
-                               int64_t npl, const int* atom_mol)
+                               int64_t npl, const int2* mol_atoms)
 {
     const int64_t npair = plist ? npl : (int64_t)nbf * (nbf + 1) / 2;
     if (npair <= 0) return hipSuccess;
@@ -392,7 +400,7 @@ hipError_t launch_one_electron(int nbf, const BFDesc* bfs, const double* pexp, c
     }
     // four wavefronts (basis-function pairs) per 256-thread block
     hipLaunchKernelGGL(one_electron_kernel, dim3((unsigned)((npair + 3) / 4)), dim3(256), 0, st, nbf, bfs, pexp,
-                       pw, natom, atoms, cx, cy, cz, boys, S, T, V, M, Lm, plist, npl, atom_mol);
+                       pw, natom, atoms, cx, cy, cz, boys, S, T, V, M, Lm, plist, npl, mol_atoms);
     return hipGetLastError();
 }
 

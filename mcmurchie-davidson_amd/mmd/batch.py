@@ -306,19 +306,27 @@ def run_batch_device(geometries, basis="6-31g", group=64, guess="core", fp32_bel
         ctx.set_molecules([len(a[4]) for a in arrs], [len(a[0]) for a in arrs])
         ctx.set_precision(fp32_below)
         P0 = None
-        if guess != "core":
+        if guess == "learned":
+            from mmd.learned import batch_learned_densities
+            dens = batch_learned_densities([(a[0], a[1], a[3], 2 * a[7]) for a in arrs], basis, guess_model, ctx)
+            P0 = np.concatenate([d.ravel() for d in dens])
+        elif guess == "sad":
+            from mmd.learned import _element_density
             blocks = []
-            for i in idx:
-                mol = Molecule(geometries[i], basis)
-                if guess == "sad":
-                    from mmd.guess import sad_density
-                    blocks.append(np.real(sad_density(mol)).ravel())
-                elif guess == "learned":
-                    from mmd.learned import learned_density
-                    blocks.append(np.real(learned_density(mol, guess_model)).ravel())
-                else:
-                    raise ValueError("guess must be 'core', 'sad' or 'learned'")
+            for a in arrs:
+                parts = [_element_density(int(z), basis) for z in a[0]]
+                n = sum(len(b) for b in parts)
+                P = np.zeros((n, n))
+                o = 0
+                for b in parts:
+                    P[o:o + len(b), o:o + len(b)] = b
+                    o += len(b)
+                if a[0].sum() != 2 * a[7]:
+                    P *= 2 * a[7] / a[0].sum()
+                blocks.append(P.ravel())
             P0 = np.concatenate(blocks)
+        elif guess != "core":
+            raise ValueError("guess must be 'core', 'sad' or 'learned'")
         res = ctx.batch_rhf([a[7] for a in arrs], [a[8] for a in arrs], P0=P0, conver=conver, acc2e=acc2e,
                             use_graph=use_graph, keep_density=keep_density)
         o = 0

@@ -145,3 +145,49 @@ def test_batch_molecule_arrays_match_molecule(oracle, basis_dir):
     assert nocc == mol.nocc
     om = oracle.OracleMolecule(geom, os.path.join(basis_dir, "6-31g.gbs"))
     assert abs(enuc - om.nuc_energy) < 1e-12
+
+
+def test_batched_learned_guess_matches_per_molecule(monkeypatch):
+    """The batch driver's learned guess (features of all clusters, one forward
+    pass, vectorised frames and assembly) equals learned_density()'s steps
+    molecule by molecule: features() and assemble() on the same network
+    output.  The network is replaced by a fixed linear map (no GPU here) and
+    the atomic blocks by fixed symmetric matrices."""
+    import numpy as np
+    from geometries import water_cluster
+    from mmd import _native, learned
+    from mmd.batch import _molecule_arrays
+    from mmd.molecule import Molecule
+    rng = np.random.default_rng(3)
+    model = learned.default_model()
+    nout = model.dims[-1]
+    Wmap = rng.standard_normal((15, nout))
+    monkeypatch.setattr(_native, "mlp_forward", lambda ctx, m, X: X.astype(np.float64) @ Wmap)
+    blocks = {}
+
+    def element_density(Z, basis):
+        if Z not in blocks:
+            n = {8: 9, 1: 2}[Z]
+            A = rng.standard_normal((n, n))
+            blocks[Z] = A + A.T
+        return blocks[Z]
+
+    geoms = [water_cluster(k, 2.0 + 1.4 * k, seed=50 + k, perturb=True) for k in (1, 2, 4)]
+    arrs = [_molecule_arrays(g, "6-31g") for g in geoms]
+    got = learned.batch_learned_densities([(a[0], a[1], a[3], 2 * a[7]) for a in arrs], "6-31g", model,
+                                          element_density=element_density)
+    for g, a, P in zip(geoms, arrs, got):
+        mol = Molecule(g, "6-31g")
+        F = learned.features(mol)
+        Fb, _ = learned.batch_features(a[0], a[1])
+        assert np.allclose(Fb, F, rtol=1e-12, atol=1e-12)
+        base = np.zeros((mol.nbasis, mol.nbasis))
+        o = 0
+        for atom in mol.atoms:
+            b = element_density(atom.charge, "6-31g")
+            base[o:o + len(b), o:o + len(b)] = b
+            o += len(b)
+        X = ((F - model.x_mean) / model.x_std).astype(np.float32)
+        corr = (X.astype(np.float64) @ Wmap) * float(model.y_std)
+        ref = learned.assemble(mol, base, corr)
+        assert np.allclose(P, ref, rtol=0, atol=1e-10)

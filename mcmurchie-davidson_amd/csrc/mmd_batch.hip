@@ -160,15 +160,119 @@ __device__ void wg_jacobi(int n, double* A, int lda, double* V, int ldv, double*
     __syncthreads();
 }
 
+constexpr int BTL = 1024;   // threads of the all-LDS molecule workgroups
+
 // LDS vectors of a molecule workgroup (the matrices are dynamic LDS).
 struct WgSmall {
     double w[BATCH_MAX_NBF];
     double cs[BATCH_MAX_NBF + 2];
     int perm[BATCH_MAX_NBF + 2];
-    double red[BT / 64];
+    double red[BTL / 64];
     double Bm[9 * 9];
     double rhs[9];
 };
+
+// Cyclic Jacobi eigensolver with A (destroyed) and V both in LDS (leading
+// dimension n).  V holds the starting basis on entry: the identity, or the
+// eigenvectors of a nearby matrix, in which case A must be V^T A0 V; on exit
+// A0 = V diag(w) V^T with V's columns in ascending order of w.  A round
+// applies the m/2 disjoint rotations of the round-robin ordering (m = n
+// rounded up to even) in one pass: each thread updates whole 2 x 2 blocks
+// (rows of one rotation, columns of another), so a round needs only two
+// barriers.  Convergence: sum of squared off-diagonal elements at most
+// 1e-30 of the squared Frobenius norm.
+__device__ void lds_jacobi(int n, double* A, double* V, double* w, int* perm, double* cs, double* red)
+{
+    const int m = (n + 1) & ~1;
+    const int h = m / 2;
+    double* c = cs;
+    double* s = cs + h;
+    double fro = 0.0;
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) fro = fma(A[e], A[e], fro);
+    fro = wg_sum(fro, red);
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        double off = 0.0;
+        for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+            const int i = e / n, j = e - i * n;
+            if (i != j) off = fma(A[e], A[e], off);
+        }
+        off = wg_sum(off, red);
+        if (!(off > 1e-30 * fro)) break;
+        for (int round = 0; round < m - 1; ++round) {
+            for (int k = threadIdx.x; k < h; k += blockDim.x) {
+                auto slot = [&](int pos) { return pos == 0 ? 0 : 1 + (pos - 1 + round) % (m - 1); };
+                int p = slot(k), q = slot(m - 1 - k);
+                if (p > q) { const int t = p; p = q; q = t; }
+                perm[2 * k] = p;
+                perm[2 * k + 1] = q;
+                double cc = 1.0, ss = 0.0;
+                if (q < n) {
+                    const double apq = A[p * n + q];
+                    if (apq != 0.0) {
+                        const double app = A[p * n + p], aqq = A[q * n + q];
+                        const double theta = (aqq - app) / (2.0 * apq);
+                        const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                        cc = 1.0 / sqrt(t * t + 1.0);
+                        ss = t * cc;
+                    }
+                }
+                c[k] = cc;
+                s[k] = ss;
+            }
+            __syncthreads();
+            // A <- J^T A J, block (k1, k2) = rows p1, q1 x columns p2, q2
+            for (int b = threadIdx.x; b < h * h; b += blockDim.x) {
+                const int k1 = b / h, k2 = b - k1 * h;
+                const double s1 = s[k1], s2 = s[k2];
+                if (s1 == 0.0 && s2 == 0.0) continue;
+                const double c1 = c[k1], c2 = c[k2];
+                const int p1 = perm[2 * k1], q1 = perm[2 * k1 + 1], p2 = perm[2 * k2], q2 = perm[2 * k2 + 1];
+                const bool r1 = q1 < n, r2 = q2 < n;   // the dummy index n has no row or column
+                const double a = A[p1 * n + p2];
+                const double bb = r2 ? A[p1 * n + q2] : 0.0;
+                const double d = r1 ? A[q1 * n + p2] : 0.0;
+                const double e = (r1 && r2) ? A[q1 * n + q2] : 0.0;
+                const double a1 = c1 * a - s1 * d, b1 = c1 * bb - s1 * e;
+                const double d1 = s1 * a + c1 * d, e1 = s1 * bb + c1 * e;
+                A[p1 * n + p2] = c2 * a1 - s2 * b1;
+                if (r2) A[p1 * n + q2] = s2 * a1 + c2 * b1;
+                if (r1) A[q1 * n + p2] = c2 * d1 - s2 * e1;
+                if (r1 && r2) A[q1 * n + q2] = s2 * d1 + c2 * e1;
+            }
+            // V <- V J
+            for (int e = threadIdx.x; e < h * n; e += blockDim.x) {
+                const int k = e / n, i = e - k * n;
+                const int p = perm[2 * k], q = perm[2 * k + 1];
+                if (q >= n || s[k] == 0.0) continue;
+                double* vr = V + i * n;
+                const double vp = vr[p], vq = vr[q];
+                vr[p] = c[k] * vp - s[k] * vq;
+                vr[q] = s[k] * vp + c[k] * vq;
+            }
+            __syncthreads();
+        }
+    }
+    // ascending order: rank of each eigenvalue (ties by index)
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double di = A[i * n + i];
+        int r = 0;
+        for (int j = 0; j < n; ++j) {
+            const double dj = A[j * n + j];
+            if (dj < di || (dj == di && j < i)) ++r;
+        }
+        perm[i] = r;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) w[perm[i]] = A[i * n + i];
+    __syncthreads();
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+        const int i = e / n, j = e - i * n;
+        A[i * n + perm[j]] = V[e];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) V[e] = A[e];
+    __syncthreads();
+}
 
 // LDS of a molecule workgroup: one n x n matrix (leading dimension n) plus
 // small vectors; n <= BATCH_MAX_NBF.
@@ -197,6 +301,37 @@ __global__ void __launch_bounds__(BT) batch_overlap_kernel(BatchArgs B)
         double x = 0.0, u = 0.0;
         for (int k = 0; k < n; ++k) {
             const double vv = V[(size_t)i * n + k] * V[(size_t)j * n + k];
+            x = fma(vv, 1.0 / sqrt(L.w[k]), x);
+            u = fma(vv, sqrt(L.w[k]), u);
+        }
+        blk(B.X, N, M.f0)[(size_t)i * N + j] = x;
+        blk(B.U, N, M.f0)[(size_t)i * N + j] = u;
+    }
+    if (threadIdx.x == 0 && !(L.w[0] > 0.0)) B.flag[blockIdx.x] = -1;   // not positive definite
+}
+
+// batch_overlap_kernel with S and its eigenvectors in LDS (two n x n
+// matrices of dynamic shared memory).
+__global__ void __launch_bounds__(BTL) batch_overlap_lds_kernel(BatchArgs B)
+{
+    extern __shared__ double dyn[];
+    __shared__ WgSmall L;
+    const BatchMol M = B.mol[blockIdx.x];
+    const int n = M.n, N = B.N;
+    double* A = dyn;
+    double* V = dyn + (size_t)n * n;
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+        const int i = e / n, j = e - i * n;
+        A[e] = blk(B.S, N, M.f0)[(size_t)i * N + j];
+        V[e] = i == j ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    lds_jacobi(n, A, V, L.w, L.perm, L.cs, L.red);
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+        const int i = e / n, j = e - i * n;
+        double x = 0.0, u = 0.0;
+        for (int k = 0; k < n; ++k) {
+            const double vv = V[i * n + k] * V[j * n + k];
             x = fma(vv, 1.0 / sqrt(L.w[k]), x);
             u = fma(vv, sqrt(L.w[k]), u);
         }
@@ -388,7 +523,7 @@ __device__ void lds_load(int n, const double* M, int N, double* L)
 // matrices (dynamic shared memory) hold P S, F P S, the error, X, F_diis, FO
 // and the eigenvectors in turn, so the products and the Jacobi rotations run
 // from LDS.  Same operations in the same order as batch_step_kernel.
-__global__ void __launch_bounds__(BT) batch_step_lds_kernel(BatchArgs B, int build, int diis, double conver)
+__global__ void __launch_bounds__(BTL) batch_step_lds_kernel(BatchArgs B, int build, int diis, double conver)
 {
     extern __shared__ double dyn[];
     __shared__ WgSmall L;
@@ -510,8 +645,22 @@ __global__ void __launch_bounds__(BT) batch_step_lds_kernel(BatchArgs B, int bui
     if (!fo_from_lds) lds_load(n, F, N, M1);
     lds_gemm(n, true, M0, false, M1, M2);
     lds_gemm(n, false, M2, false, M0, M1);                         // FO in M1
-    wg_jacobi(n, M1, n, M2, n, L.w, L.perm, L.cs, L.red);          // C' in M2 (LDS)
-    lds_gemm(n, false, M0, false, M2, M1);                         // C = X C' in M1
+    // eigenvectors of FO, started from the last step's (kept in the work
+    // area): A = V^T FO V is nearly diagonal once the SCF settles
+    double* Vkeep = B.work + M.wo;
+    if (Sx.warm) {
+        lds_load(n, Vkeep, n, M0);
+        lds_gemm(n, false, M1, false, M0, M2);                     // FO V
+        lds_gemm(n, true, M0, false, M2, M1);                      // V^T FO V
+    } else {
+        for (int e = threadIdx.x; e < n * n; e += blockDim.x) M0[e] = (e / n == e % n) ? 1.0 : 0.0;
+        __syncthreads();
+    }
+    lds_jacobi(n, M1, M0, L.w, L.perm, L.cs, L.red);               // C' in M0
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) Vkeep[e] = M0[e];
+    if (threadIdx.x == 0) Sx.warm = 1;
+    lds_load(n, X, N, M2);
+    lds_gemm(n, false, M2, false, M0, M1);                         // C = X C' in M1
     const int nocc = M.nocc;
     double e_el = 0.0, rms = 0.0;
     for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
@@ -565,10 +714,42 @@ __global__ void batch_count_kernel(int nmol, const BatchState* state, int* out)
     }
 }
 
+// Copies each molecule's n x n block between the N x N block-diagonal
+// matrix and the packed array (unpack: packed -> full).
+__global__ void batch_pack_kernel(BatchArgs B, double* full, double* packed, int unpack)
+{
+    const BatchMol M = B.mol[blockIdx.x];
+    const int n = M.n;
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+        const size_t o = (size_t)(M.f0 + e / n) * B.N + M.f0 + e % n;
+        if (unpack)
+            full[o] = packed[M.po + e];
+        else
+            packed[M.po + e] = full[o];
+    }
+}
+
 }  // namespace
+
+hipError_t launch_batch_pack(const BatchArgs& B, const double* full, double* packed, hipStream_t st)
+{
+    hipLaunchKernelGGL(batch_pack_kernel, dim3(B.nmol), dim3(256), 0, st, B, const_cast<double*>(full), packed, 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_unpack(const BatchArgs& B, const double* packed, double* full, hipStream_t st)
+{
+    hipLaunchKernelGGL(batch_pack_kernel, dim3(B.nmol), dim3(256), 0, st, B, full, const_cast<double*>(packed), 1);
+    return hipGetLastError();
+}
 
 hipError_t launch_batch_overlap(const BatchArgs& B, hipStream_t st)
 {
+    if (B.lds_bytes >= 2 * (size_t)B.maxn * B.maxn * sizeof(double) + sizeof(WgSmall)) {
+        const size_t dyn = 2 * (size_t)B.maxn * B.maxn * sizeof(double);
+        hipLaunchKernelGGL(batch_overlap_lds_kernel, dim3(B.nmol), dim3(BTL), dyn, st, B);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(batch_overlap_kernel, dim3(B.nmol), dim3(BT), 0, st, B);
     return hipGetLastError();
 }
@@ -579,7 +760,7 @@ hipError_t launch_batch_step(const BatchArgs& B, int build, int diis, double con
     // beside the small vectors; the global-memory step otherwise
     if (B.lds_bytes >= 3 * (size_t)B.maxn * B.maxn * sizeof(double) + sizeof(WgSmall)) {
         const size_t dyn = 3 * (size_t)B.maxn * B.maxn * sizeof(double);
-        hipLaunchKernelGGL(batch_step_lds_kernel, dim3(B.nmol), dim3(BT), dyn, st, B, build, diis, conver);
+        hipLaunchKernelGGL(batch_step_lds_kernel, dim3(B.nmol), dim3(BTL), dyn, st, B, build, diis, conver);
     } else {
         hipLaunchKernelGGL(batch_step_kernel, dim3(B.nmol), dim3(BT), 0, st, B, build, diis, conver);
     }

@@ -342,11 +342,20 @@ void molecule_shell_ranges(mmd_ctx* c)
         if (c->mol_s0[m] > c->mol_s0[m + 1]) c->mol_s0[m] = c->mol_s0[m + 1];
 }
 
+// Host view of the Schwarz diagonal (pq|pq): element (i, j) = p[row[i] + j].
+// The full N x N matrix, or the molecules' diagonal blocks packed one after
+// another (pairs never mix molecules).
+struct DiagView {
+    const double* p = nullptr;
+    std::vector<int64_t> row;
+    double at(int i, int j) const { return p[row[i] + j]; }
+};
+
 // Shell pairs of each molecule, grouped by pair class and sorted in each
 // class by (molecule, primitive pairs descending, Schwarz factor
 // descending), with their primitive-pair tables.  diag: the full N x N
 // Schwarz diagonal (pq|pq), or null before it is known (factors 1).
-int build_pairs(mmd_ctx* c, const std::vector<double>* diag)
+int build_pairs(mmd_ctx* c, const DiagView* diag)
 {
     const int N = c->nbf;
     molecule_shell_ranges(c);
@@ -391,7 +400,7 @@ int build_pairs(mmd_ctx* c, const std::vector<double>* diag)
                     q = 0.0;
                     for (int x = 0; x < ncart(A.l); ++x)
                         for (int y = 0; y < ncart(B.l); ++y) {
-                            const double v = (*diag)[(size_t)(A.bf0 + x) * N + B.bf0 + y];
+                            const double v = diag->at(A.bf0 + x, B.bf0 + y);
                             const double r = v > 0.0 ? sqrt(v) : 0.0;
                             if (r > q) q = r;
                         }
@@ -778,20 +787,32 @@ int collect_profile(mmd_ctx* c)
 namespace {
 // Install a Schwarz diagonal (N*N, (pq|pq)) and rebuild the sorted shell-pair
 // tables with their factors q = max sqrt((ab|ab)).
+// The Schwarz matrix (pq|pq) is in d_schwarz; diag views it on the host for
+// the pair factors.
+int finish_schwarz(mmd_ctx* ctx, const DiagView& diag)
+{
+    const size_t nn = (size_t)ctx->nbf * ctx->nbf;
+    HIP_TRY(launch_sqrt_clamped(nn, ctx->d_schwarz.p, ctx->d_sqS.p, ctx->st));
+    HIP_TRY(hipStreamSynchronize(ctx->st));
+    ctx->schwarz_ok = true;
+    if (!ctx->shells_ok) return MMD_OK;   // only the per-function paths are available
+    const int rc = build_pairs(ctx, &diag);   // pair factors q from the diagonal
+    if (rc) return rc;
+    // the Fock build's work units belong to the pair table: built here, at
+    // set-up, so that a build (mmd_fock_delta_noalloc) allocates nothing
+    return build_units(ctx);
+}
+
 int set_schwarz(mmd_ctx* ctx, const std::vector<double>& h)
 {
     const int N = ctx->nbf;
     const size_t nn = (size_t)N * N;
     HIP_TRY(hipMemcpyAsync(ctx->d_schwarz.p, h.data(), sizeof(double) * nn, hipMemcpyHostToDevice, ctx->st));
-    HIP_TRY(launch_sqrt_clamped(nn, ctx->d_schwarz.p, ctx->d_sqS.p, ctx->st));
-    HIP_TRY(hipStreamSynchronize(ctx->st));
-    ctx->schwarz_ok = true;
-    if (!ctx->shells_ok) return MMD_OK;   // only the per-function paths are available
-    const int rc = build_pairs(ctx, &h);   // pair factors q from the diagonal
-    if (rc) return rc;
-    // the Fock build's work units belong to the pair table: built here, at
-    // set-up, so that a build (mmd_fock_delta_noalloc) allocates nothing
-    return build_units(ctx);
+    DiagView v;
+    v.p = h.data();
+    v.row.resize(N);
+    for (int i = 0; i < N; ++i) v.row[i] = (int64_t)i * N;
+    return finish_schwarz(ctx, v);
 }
 
 }  // namespace
@@ -1572,6 +1593,7 @@ int mmd_batch_rhf(mmd_ctx* ctx, const int32_t* nocc, const double* enuc, const d
         b.pad = 0;
         b.wo = wtot;
         b.d8 = dtot;
+        b.po = ptot;
         b.enuc = enuc[m];
         wtot += 3 * (size_t)n * n;
         dtot += 16 * (size_t)n * n;
@@ -1587,10 +1609,8 @@ int mmd_batch_rhf(mmd_ctx* ctx, const int32_t* nocc, const double* enuc, const d
                                 M > 1 ? c->d_mol_atoms.p : nullptr));
     for (DevBuf<double>* b : {&c->b_H, &c->b_X, &c->b_U, &c->b_F, &c->b_P, &c->b_Pold, &c->b_dP, &c->b_G})
         HIP_TRY(b->reserve(nn));
-    HIP_TRY(c->b_work.reserve(std::max<size_t>(1, wtot)));
     HIP_TRY(c->b_diis.reserve(std::max<size_t>(1, dtot)));
     HIP_TRY(c->b_mo.reserve(N));
-    HIP_TRY(c->b_mol.reserve(M));
     HIP_TRY(c->b_state.reserve(M));
     HIP_TRY(c->b_flag.reserve(M));
     HIP_TRY(c->b_count.reserve(1));
@@ -1601,11 +1621,28 @@ int mmd_batch_rhf(mmd_ctx* ctx, const int32_t* nocc, const double* enuc, const d
     HIP_TRY(c->d_schwarz.reserve(nn));
     HIP_TRY(launch_eri_diag(N, c->d_bf.p, c->d_bf_exp.p, c->d_bf_w.p, c->d_boys.p, c->d_schwarz.p, st,
                             M > 1 ? c->d_plist.p : nullptr, c->npl));
+    HIP_TRY(c->b_mol.reserve(M));
+    HIP_TRY(hipMemcpyAsync(c->b_mol.p, mols.data(), sizeof(BatchMol) * M, hipMemcpyHostToDevice, st));
+    BatchArgs B;
+    B.nmol = M;
+    B.N = N;
+    B.maxn = c->rw;
+    B.lds_bytes = c->lds_per_block;
+    B.mol = c->b_mol.p;
+    HIP_TRY(c->b_work.reserve(std::max<size_t>(1, wtot)));   // 3 n^2 per molecule: staging for packed blocks
     {
-        std::vector<double> h(nn);
-        HIP_TRY(hipMemcpyAsync(h.data(), c->d_schwarz.p, sizeof(double) * nn, hipMemcpyDeviceToHost, st));
+        // the molecules' diagonal blocks of (pq|pq), packed, for the pair factors
+        HIP_TRY(launch_batch_pack(B, c->d_schwarz.p, c->b_work.p, st));
+        std::vector<double> h(std::max<size_t>(1, ptot));
+        HIP_TRY(hipMemcpyAsync(h.data(), c->b_work.p, sizeof(double) * ptot, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
-        if ((rc = set_schwarz(c, h))) return rc;
+        DiagView v;
+        v.p = h.data();
+        v.row.resize(N);
+        for (int m = 0; m < M; ++m)
+            for (int i = 0; i < mols[m].n; ++i)
+                v.row[mols[m].f0 + i] = (int64_t)mols[m].po + (int64_t)i * mols[m].n - mols[m].f0;
+        if ((rc = finish_schwarz(c, v))) return rc;
     }
     // buffers of the Fock build, reserved before any capture
     HIP_TRY(c->d_rowctr.reserve(2 * NQCLASS));
@@ -1628,17 +1665,10 @@ int mmd_batch_rhf(mmd_ctx* ctx, const int32_t* nocc, const double* enuc, const d
         x.energy = 0.0;
         x.prms = 1e20;
     }
-    HIP_TRY(hipMemcpyAsync(c->b_mol.p, mols.data(), sizeof(BatchMol) * M, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(c->b_state.p, st0.data(), sizeof(BatchState) * M, hipMemcpyHostToDevice, st));
     HIP_TRY(launch_fill_int(M, c->b_flag.p, 0, st));
     HIP_TRY(hipMemcpyAsync(c->b_F.p, c->b_H.p, sizeof(double) * nn, hipMemcpyDeviceToDevice, st));
     HIP_TRY(launch_fill(nn, c->b_P.p, 0.0, st));
-    BatchArgs B;
-    B.nmol = M;
-    B.N = N;
-    B.maxn = c->rw;
-    B.lds_bytes = c->lds_per_block;
-    B.mol = c->b_mol.p;
     B.state = c->b_state.p;
     B.S = S;
     B.H = c->b_H.p;
@@ -1656,15 +1686,9 @@ int mmd_batch_rhf(mmd_ctx* ctx, const int32_t* nocc, const double* enuc, const d
     HIP_TRY(launch_batch_overlap(B, st));
     if (P0) {
         // an initial density (SAD, learned): F = H + G(P0), P_old = P0
-        std::vector<double> full(nn, 0.0);
-        size_t o = 0;
-        for (int m = 0; m < M; ++m) {
-            const int n = c->mol_n[m], f0 = c->mol_f0[m];
-            for (int i = 0; i < n; ++i)
-                for (int j = 0; j < n; ++j) full[(size_t)(f0 + i) * N + f0 + j] = P0[o + (size_t)i * n + j];
-            o += (size_t)n * n;
-        }
-        HIP_TRY(hipMemcpyAsync(c->b_P.p, full.data(), sizeof(double) * nn, hipMemcpyHostToDevice, st));
+        // (packed blocks through the work area, which the steps use later)
+        HIP_TRY(hipMemcpyAsync(c->b_work.p, P0, sizeof(double) * ptot, hipMemcpyHostToDevice, st));
+        HIP_TRY(launch_batch_unpack(B, c->b_work.p, c->b_P.p, st));
         HIP_TRY(hipStreamSynchronize(st));
         if ((rc = enqueue_fock_sym(c, c->b_P.p, acc2e, c->b_G.p))) return rc;
         hipLaunchKernelGGL(batch_axpy_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, nn, c->b_H.p,
@@ -1726,24 +1750,16 @@ int mmd_batch_rhf(mmd_ctx* ctx, const int32_t* nocc, const double* enuc, const d
     HIP_TRY(hipMemcpyAsync(out.data(), c->b_state.p, sizeof(BatchState) * M, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(fl.data(), c->b_flag.p, sizeof(int) * M, hipMemcpyDeviceToHost, st));
     if (mo_out) HIP_TRY(hipMemcpyAsync(mo_out, c->b_mo.p, sizeof(double) * N, hipMemcpyDeviceToHost, st));
-    std::vector<double> full;
     if (P_out) {
-        full.resize(nn);
-        HIP_TRY(hipMemcpyAsync(full.data(), c->b_P.p, sizeof(double) * nn, hipMemcpyDeviceToHost, st));
+        HIP_TRY(launch_batch_pack(B, c->b_P.p, c->b_work.p, st));
+        HIP_TRY(hipMemcpyAsync(P_out, c->b_work.p, sizeof(double) * ptot, hipMemcpyDeviceToHost, st));
     }
     HIP_TRY(hipStreamSynchronize(st));
-    size_t o = 0;
     for (int m = 0; m < M; ++m) {
         if (fl[m] == -2) return fail(MMD_ERR_STATE, "molecule %d: singular DIIS matrix", m);
         energy[m] = out[m].energy;
         iterations[m] = out[m].iterations;
         converged[m] = out[m].converged;
-        if (P_out) {
-            const int n = c->mol_n[m], f0 = c->mol_f0[m];
-            for (int i = 0; i < n; ++i)
-                for (int j = 0; j < n; ++j) P_out[o + (size_t)i * n + j] = full[(size_t)(f0 + i) * N + f0 + j];
-            o += (size_t)n * n;
-        }
     }
     return MMD_OK;
 }

@@ -283,59 +283,113 @@ def _molecule_arrays(geometry, basis):
     return Z, xyz, origin, lmn, nprim, exps, w, nocc, enuc
 
 
+def _device_group(ctx, geometries, idx, basis, guess, fp32_below, conver, acc2e, use_graph, keep_density,
+                  guess_model, out):
+    """One group of run_batch_device on ctx: results into out[i] for i in idx."""
+    arrs = [_molecule_arrays(geometries[i], basis) for i in idx]
+    ctx.set_basis(np.concatenate([a[2] for a in arrs]), np.concatenate([a[3] for a in arrs]),
+                  np.concatenate([a[4] for a in arrs]), np.concatenate([a[5] for a in arrs]),
+                  np.concatenate([a[6] for a in arrs]))
+    ctx.set_atoms(np.concatenate([a[0] for a in arrs]).astype(np.float64),
+                  np.concatenate([a[1] for a in arrs]))
+    ctx.set_molecules([len(a[4]) for a in arrs], [len(a[0]) for a in arrs])
+    ctx.set_precision(fp32_below)
+    P0 = None
+    if guess == "learned":
+        from mmd.learned import batch_learned_densities
+        dens = batch_learned_densities([(a[0], a[1], a[3], 2 * a[7]) for a in arrs], basis, guess_model, ctx)
+        P0 = np.concatenate([d.ravel() for d in dens])
+    elif guess == "sad":
+        from mmd.learned import _element_density
+        blocks = []
+        for a in arrs:
+            parts = [_element_density(int(z), basis) for z in a[0]]
+            n = sum(len(b) for b in parts)
+            P = np.zeros((n, n))
+            o = 0
+            for b in parts:
+                P[o:o + len(b), o:o + len(b)] = b
+                o += len(b)
+            if a[0].sum() != 2 * a[7]:
+                P *= 2 * a[7] / a[0].sum()
+            blocks.append(P.ravel())
+        P0 = np.concatenate(blocks)
+    res = ctx.batch_rhf([a[7] for a in arrs], [a[8] for a in arrs], P0=P0, conver=conver, acc2e=acc2e,
+                        use_graph=use_graph, keep_density=keep_density)
+    o = 0
+    for k, i in enumerate(idx):
+        n = len(arrs[k][4])
+        r = {"energy": float(res[0][k]), "iterations": int(res[1][k]), "converged": bool(res[2][k]),
+             "nbasis": n}
+        if keep_density:
+            r["P"] = res[3][o:o + n * n].reshape(n, n).copy()
+        o += n * n
+        out[i] = r
+
+
+_DEVICE_CTXS = {}
+_DEVICE_CTX_LOCK = threading.Lock()
+
+
+def _device_contexts(device, count):
+    """`count` contexts on `device`, kept for the process (each holds its own
+    stream and device buffers)."""
+    with _DEVICE_CTX_LOCK:
+        lst = _DEVICE_CTXS.setdefault(device, [])
+        while len(lst) < count:
+            lst.append(_native.Context(device))
+        return lst[:count]
+
+
 def run_batch_device(geometries, basis="6-31g", group=64, guess="core", fp32_below=0.0, conver=1e-8,
-                     acc2e=1e-12, device=0, use_graph=True, keep_density=False, guess_model=None, ctx=None):
+                     acc2e=1e-12, device=0, use_graph=True, keep_density=False, guess_model=None, ctx=None,
+                     streams=2):
     """Direct RHF energies of all geometries, `group` molecules at a time in
     one block-diagonal basis on the GPU (mmd_batch_rhf).  guess: 'core' (the
-    reference's), 'sad' or 'learned' (the initial densities are formed per
-    molecule on the host side first).  fp32_below: the FP32 tier of the Fock
-    builds.  Returns one dict per geometry: energy, iterations, converged,
-    nbasis (and the density P with keep_density)."""
+    reference's), 'sad' or 'learned' (initial densities formed for the whole
+    group on the host side, the network on the GPU).  fp32_below: the FP32
+    tier of the Fock builds.  streams: groups in flight at once, each on its
+    own context (stream), so that one group's host-side set-up overlaps
+    another's device work; with ctx given, that context alone.  Returns one
+    dict per geometry: energy, iterations, converged, nbasis (and the density
+    P with keep_density)."""
     if group < 1:
         raise ValueError("group must be >= 1")
-    ctx = ctx if ctx is not None else _native.Context(device)
+    if guess not in ("core", "sad", "learned"):
+        raise ValueError("guess must be 'core', 'sad' or 'learned'")
+    if guess == "learned":   # the network's weights once, before any worker needs them
+        from mmd.learned import default_model
+        guess_model = guess_model or default_model()
+    groups = [list(range(g0, min(len(geometries), g0 + group))) for g0 in range(0, len(geometries), group)]
+    ctxs = [ctx] if ctx is not None else _device_contexts(device, max(1, min(int(streams), len(groups))))
     out = [None] * len(geometries)
-    for g0 in range(0, len(geometries), group):
-        idx = list(range(g0, min(len(geometries), g0 + group)))
-        arrs = [_molecule_arrays(geometries[i], basis) for i in idx]
-        ctx.set_basis(np.concatenate([a[2] for a in arrs]), np.concatenate([a[3] for a in arrs]),
-                      np.concatenate([a[4] for a in arrs]), np.concatenate([a[5] for a in arrs]),
-                      np.concatenate([a[6] for a in arrs]))
-        ctx.set_atoms(np.concatenate([a[0] for a in arrs]).astype(np.float64),
-                      np.concatenate([a[1] for a in arrs]))
-        ctx.set_molecules([len(a[4]) for a in arrs], [len(a[0]) for a in arrs])
-        ctx.set_precision(fp32_below)
-        P0 = None
-        if guess == "learned":
-            from mmd.learned import batch_learned_densities
-            dens = batch_learned_densities([(a[0], a[1], a[3], 2 * a[7]) for a in arrs], basis, guess_model, ctx)
-            P0 = np.concatenate([d.ravel() for d in dens])
-        elif guess == "sad":
-            from mmd.learned import _element_density
-            blocks = []
-            for a in arrs:
-                parts = [_element_density(int(z), basis) for z in a[0]]
-                n = sum(len(b) for b in parts)
-                P = np.zeros((n, n))
-                o = 0
-                for b in parts:
-                    P[o:o + len(b), o:o + len(b)] = b
-                    o += len(b)
-                if a[0].sum() != 2 * a[7]:
-                    P *= 2 * a[7] / a[0].sum()
-                blocks.append(P.ravel())
-            P0 = np.concatenate(blocks)
-        elif guess != "core":
-            raise ValueError("guess must be 'core', 'sad' or 'learned'")
-        res = ctx.batch_rhf([a[7] for a in arrs], [a[8] for a in arrs], P0=P0, conver=conver, acc2e=acc2e,
-                            use_graph=use_graph, keep_density=keep_density)
-        o = 0
-        for k, i in enumerate(idx):
-            n = len(arrs[k][4])
-            r = {"energy": float(res[0][k]), "iterations": int(res[1][k]), "converged": bool(res[2][k]),
-                 "nbasis": n}
-            if keep_density:
-                r["P"] = res[3][o:o + n * n].reshape(n, n).copy()
-            o += n * n
-            out[i] = r
+    args = (basis, guess, fp32_below, conver, acc2e, use_graph, keep_density, guess_model, out)
+    if len(ctxs) == 1:
+        for idx in groups:
+            _device_group(ctxs[0], geometries, idx, *args)
+        return out
+    todo = list(reversed(groups))
+    lock = threading.Lock()
+    errors = []
+
+    def worker(c):
+        while True:
+            with lock:
+                if not todo or errors:
+                    return
+                idx = todo.pop()
+            try:
+                _device_group(c, geometries, idx, *args)
+            except BaseException as e:   # re-raised in the caller
+                with lock:
+                    errors.append(e)
+                return
+
+    threads = [threading.Thread(target=worker, args=(c,)) for c in ctxs]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    if errors:
+        raise errors[0]
     return out

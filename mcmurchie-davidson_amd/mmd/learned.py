@@ -20,6 +20,7 @@ backward passes and its training run in libmmd's HIP kernels
 densities that the GPU SCF itself produces (``make_dataset``).
 """
 import os
+import threading
 
 import numpy as np
 
@@ -213,6 +214,9 @@ def learned_density(mol, model=None):
     return assemble(mol, sad_density(mol), predict(mol, model))
 
 
+_LOCK = threading.Lock()
+
+
 def _element_density(Z, basis):
     """SAD half density of a free atom of element Z in ``basis`` (cached)."""
     from mmd.guess import atomic_density
@@ -223,7 +227,8 @@ def _element_density(Z, basis):
     class _Mol:
         basis_name = basis
 
-    return atomic_density(_Mol, _Atom)
+    with _LOCK:
+        return atomic_density(_Mol, _Atom)
 
 
 def batch_features(Z, xyz):
@@ -296,7 +301,8 @@ def batch_learned_densities(molecules, basis, model=None, ctx=None, element_dens
         feats.append(F)
         frames.append(R)
     X = ((np.concatenate(feats) - model.x_mean) / model.x_std).astype(np.float32)
-    Y = _native.mlp_forward(ctx, model, X).astype(float) * float(model.y_std)
+    with _LOCK:   # the network's device buffers are shared
+        Y = _native.mlp_forward(ctx, model, X).astype(float) * float(model.y_std)
     out = []
     k = 0
     for (Z, xyz, lmn, nelec), R in zip(molecules, frames):

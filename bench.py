@@ -1,17 +1,23 @@
 #!/usr/bin/env python
-"""Benchmark: converged direct FP64 RHF of (H2O)8 / cc-pVDZ on one MI355X.
+"""Benchmark: converged SCF/s of 512 small water clusters on one MI355X.
 
-Workload: 8 waters at the README monomer geometry, each rotated uniformly at
-random and placed in a 7 A cube with O-O >= 2.7 A (seed 0); cc-pVDZ with
-Cartesian d (200 functions); direct SCF with Schwarz threshold 1e-12, all
-FP64 on the GPU.  One step = one `Molecule(geometry, 'cc-pvdz').RHF(direct=True)`
-call from the geometry string to a converged energy (one-electron integrals,
-S^-1/2 and the Schwarz pass included).
+Default workload (the headline): 512 x (H2O)n with n uniform in 2..6
+(perturbed monomers, random orientations, seed 1), 6-31G, direct RHF to
+||dP|| < 1e-8.  One step = all 512 molecules from geometry strings to
+converged energies in the full form: batched on the device (groups of 64
+molecules in one block-diagonal basis, two groups in flight, each SCF
+iteration replayed as a HIP graph), learned initial density, FP32 tier for
+shell quartets with bound below 1e-4.  Every molecule's energy is checked
+against the batched FP64 core-guess run (|dE| <= 1 uEh), which is checked
+against the plain FP64 path one molecule at a time; the plain path's rate is
+measured beside it.
 
-Printed (one JSON line): converged SCF/s (value), median Fock-build ms,
-quartets/s, the roofline of the dominant Fock-build class kernel (model
-FLOPs / HIP-event time) and a CPU baseline from the oracle on a bounded
-sample of the same workload.
+Printed (one JSON line): SCF/s (value), the roofline of the dominant class
+kernel (model FLOPs per launch over its HIP-event time, from the profiled
+last timed step), (H2O)8 / cc-pVDZ Fock-build ms and quartets/s as extra
+keys, and a CPU baseline from the oracle on a bounded sample of the same
+workload, rebased on formPT's survivors per iteration.  Other workloads:
+--workload h2o8 | h2o16-tier | batch512 | ao2mo.
 """
 import argparse
 import json
@@ -44,6 +50,10 @@ TRAFFIC = {
     (2, 1, 1, 1): 7.645e+06, (2, 1, 2, 0): 5.551e+06, (2, 1, 2, 1): 6.124e+06, (2, 2, 0, 0): 3.453e+06,
     (2, 2, 1, 0): 5.270e+06, (2, 2, 1, 1): 2.754e+06, (2, 2, 2, 0): 2.329e+06, (2, 2, 2, 1): 5.559e+06,
 }
+
+
+# The same for the dominant class kernel of the headline workload.
+TRAFFIC_HEADLINE = {}
 
 
 def ncart(l):
@@ -269,31 +279,234 @@ def batch_geometries(n=512, seed=1):
     return [water_cluster(int(k), 2.0 + 1.4 * int(k), seed=1000 + i, perturb=True) for i, k in enumerate(ns)]
 
 
-def bench_batch_device(args):
+def _profile_sums(ctxs):
+    """Per-class sums of the profiled builds of all contexts: {class: [nq64,
+    nq32, np64, np32, ms64, ms32, launches]}."""
+    per = {}
+    for c in ctxs:
+        lc, nq, npr, ms, nb = c.fock_profile_sums()
+        for k in range(len(nb)):
+            if nb[k] == 0:
+                continue
+            key = tuple(int(x) for x in lc[k])
+            acc = per.setdefault(key, [0, 0, 0, 0, 0.0, 0.0, 0])
+            for t in range(2):
+                acc[t] += int(nq[k, t])
+                acc[2 + t] += int(npr[k, t])
+                acc[4 + t] += float(ms[k, t])
+            acc[6] += int(nb[k])
+    return per
+
+
+def headline_roofline(per):
+    """Roofline of the dominant class kernel (most FP64 event time): model
+    FLOPs of its surviving primitive quartets (plus 12 per surviving basis-
+    function quartet for the digestion) per launch over the launch's event
+    time; FP64 peak."""
+    def flops(key, t):
+        la, lb, lc, ld = key
+        return per[key][2 + t] * class_flops(*key) + 12.0 * per[key][t] * ncart(la) * ncart(lb) * ncart(lc) * ncart(ld)
+    dom = max(per, key=lambda k: per[k][4])
+    v = per[dom]
+    achieved = flops(dom, 0) / (v[4] * 1e-3) / 1e12
+    tot_f = sum(flops(k, t) for k in per for t in range(2))
+    tot_t = sum(per[k][4] + per[k][5] for k in per)
+    return {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": TRAFFIC_HEADLINE.get(dom),
+            "kernel": "eri_fock_class_kernel<%d,%d,%d,%d,double>" % dom,
+            "kernel_avg_ms": v[4] / v[6], "launches": v[6],
+            "model_flops_per_launch": flops(dom, 0) / v[6],
+            "prim_quartets_per_launch": v[2] / v[6], "shell_quartets_per_launch": v[0] / v[6],
+            "all_classes_achieved": tot_f / (tot_t * 1e-3) / 1e12,
+            "timing": "HIP events on the class's stream around its FP64 launch, profiled timed step"}
+
+
+def _record_builds(mol):
+    """Wrap mol._direct_G so that every Fock build's real dP = P - P_old is kept."""
+    import numpy as np
+    mol._dps = []
+    orig = mol._direct_G
+
+    def rec(P, P_old):
+        mol._dps.append(np.real(np.asarray(P) - np.asarray(P_old)).copy())
+        return orig(P, P_old)
+    mol._direct_G = rec
+    return mol
+
+
+def cpu_baseline_batch(geoms, seconds):
+    """The oracle (C restatement of formPT, cython/fock.pyx:10-87, with the
+    reference's per-quartet ERI) timed on the host cores, rebased on formPT's
+    survivors per iteration.
+
+    One molecule of each size n = 2..6 of the workload runs the plain GPU SCF
+    (reference iteration semantics) with every build's dP recorded.  For each
+    build the exact number of quartets formPT keeps is counted (no integrals)
+    and windows of formPT (limit / start_ij: L consecutive canonical quartets
+    from a random ij row, survivors computed with their ERIs) are timed; a
+    least-squares fit t = a visited + c survivors over the windows prices the
+    builds.  An SCF = one-electron integrals + Schwarz pass (timed whole) +
+    its builds.  The 512-molecule time is the sum over the size mix.  The
+    all-cores leg times the same windows on every allowed core at once
+    (threads; the oracle's C calls release the GIL) and divides the total
+    over the cores, as independent molecules would run."""
+    import concurrent.futures as cf
+    import numpy as np
+    import oracle
+    from mmd.molecule import Molecule
+    gbs = os.path.join(PKG, "mmd", "basis", "6-31g.gbs")
+    n_of = [len([ln for ln in g.split("\n")[1:] if ln.strip()]) // 3 for g in geoms]
+    sizes = sorted(set(n_of))
+    rng = np.random.default_rng(77)
+    jobs = []      # (size, om, screen, dP, start_ij, L)
+    mols = {}
+    for n in sizes:
+        i = n_of.index(n)
+        m = _record_builds(Molecule(geoms[i], "6-31g"))
+        m.RHF(doPrint=False, direct=True)
+        om = oracle.OracleMolecule(geoms[i], gbs)
+        t0 = time.perf_counter()
+        om.one_electron()
+        scr = om.schwarz()
+        t_setup = time.perf_counter() - t0
+        N = om.nbasis
+        npair = N * (N + 1) // 2
+        visited = npair * (npair + 1) // 2
+        surv = [oracle.form_pt_survivors(dp, scr, 1e-12) for dp in m._dps]
+        mols[n] = {"N": N, "builds": len(m._dps), "visited": visited, "survivors": surv, "setup_s": t_setup,
+                   "iterations": int(m.iterations)}
+        for b, dp in enumerate(m._dps):
+            for _ in range(3):
+                jobs.append((n, b, om, scr, dp, int(rng.integers(0, npair)), 1500))
+
+    def run_window(job):
+        n, b, om, scr, dp, start, L = job
+        t0 = time.perf_counter()
+        _, sv = om.form_pt(dp.astype(complex), np.zeros_like(dp, dtype=complex), scr, 1e-12, limit=L, start_ij=start)
+        el = time.perf_counter() - t0
+        # visited = quartets from row `start` up to L (fewer at the last rows)
+        N = mols[n]["N"]
+        npair = N * (N + 1) // 2
+        avail = sum(ij + 1 for ij in range(start, npair)) if npair - start < 4 * L else L
+        return n, min(L, avail), sv, el
+
+    def fit(res):
+        cost = {}
+        for n in sizes:
+            r = [(v, sv, el) for (nn, v, sv, el) in res if nn == n]
+            A = np.array([[v, sv] for v, sv, _ in r], dtype=float)
+            y = np.array([el for _, _, el in r])
+            coef, *_ = np.linalg.lstsq(A, y, rcond=None)
+            a, c = float(coef[0]), float(coef[1])
+            if a < 0 or c <= 0:   # degenerate sample: price every quartet by its survivors only
+                a, c = 0.0, float(y.sum() / max(1.0, A[:, 1].sum()))
+            info = mols[n]
+            t_scf = info["setup_s"] + sum(a * info["visited"] + c * s for s in info["survivors"])
+            cost[n] = (t_scf, a, c)
+        return cost
+
+    # 1 core
+    t0 = time.perf_counter()
+    res1 = []
+    for job in jobs:
+        res1.append(run_window(job))
+        if time.perf_counter() - t0 > 3 * seconds:
+            break
+    single_wall = time.perf_counter() - t0
+    cost1 = fit(res1)
+    total1 = sum(cost1[n][0] for n in n_of)
+    # all allowed cores at once
+    try:
+        ncores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncores = os.cpu_count() or 1
+    ncores = max(1, min(16, ncores))
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=ncores) as ex:
+        resP = list(ex.map(run_window, jobs[:len(res1)]))
+    par_wall = time.perf_counter() - t0
+    costP = fit(resP)
+    totalP = sum(costP[n][0] for n in n_of) / ncores
+    return {
+        "value": len(geoms) / total1, "unit": "SCF/s", "cores": 1, "kind": "port",
+        "sample": ("oracle formPT windows (%d windows of <= 1500 canonical quartets at random ij rows, with ERIs) "
+                   "on the recorded dP of every build of one molecule per size n=%s (plain GPU SCF, reference "
+                   "iteration semantics), priced by t = a visited + c survivors with the exact survivor count of "
+                   "each build; + one-electron and Schwarz timed whole; summed over the 512-molecule size mix "
+                   "(%.1f s of 1-core CPU)") % (len(res1), ",".join(map(str, sizes)), single_wall),
+        "nproc": os.cpu_count(),
+        "all_cores": {"value": len(geoms) / totalP, "cores": ncores, "wall_s": par_wall,
+                      "how": "the same windows on %d threads at once; total / cores" % ncores},
+        "per_size": {str(n): {"nbasis": mols[n]["N"], "builds": mols[n]["builds"],
+                              "iterations": mols[n]["iterations"],
+                              "survivors_per_build": mols[n]["survivors"],
+                              "visited_per_build": mols[n]["visited"],
+                              "cpu_s_per_scf_1core": cost1[n][0], "us_per_survivor_1core": 1e6 * cost1[n][2]}
+                     for n in sizes},
+    }
+
+
+def cfg2_extras(args):
+    """(H2O)8 / cc-pVDZ FP64 direct RHF (seed 0, 7 A cube): median Fock-build
+    ms over iterations 2..n and quartets/s (unique basis-function quartets
+    per build over the build time)."""
+    import numpy as np
+    from mmd.workloads import water_cluster
+    geom = water_cluster(8, 7.0, seed=0)
+    run_scf(geom, "cc-pvdz")
+    t0 = time.perf_counter()
+    m = run_scf(geom, "cc-pvdz")
+    el = time.perf_counter() - t0
+    N = m.nbasis
+    npair = N * (N + 1) // 2
+    nquart = npair * (npair + 1) // 2
+    fms = float(np.median(m.fock_ms[1:]))
+    return {"workload": "(H2O)8 cc-pVDZ direct FP64 RHF", "n_basis": N, "fock_build_ms": fms,
+            "quartets_per_s": nquart / (fms * 1e-3), "unique_bf_quartets": nquart, "scf_s": el,
+            "energy": float(m.energy.real), "iterations": int(m.iterations)}
+
+
+def bench_headline(args):
     """The headline workload: 512 x (H2O)n, n = 2..6, 6-31G.  Full form =
     batched on the device (mmd_batch_rhf, groups of `--group` molecules in one
-    block-diagonal basis, graph-replayed iterations) + learned start + FP32
-    tier below 1e-4, from the geometry strings to converged energies.  Checked
-    per molecule against the batched FP64 core-guess run (|dE| <= 1 uEh), and
-    compared with the plain FP64 path one molecule at a time
-    (Molecule(g).RHF(direct=True)) on every 8th geometry."""
+    block-diagonal basis, graph-replayed iterations, two groups in flight) +
+    learned start + FP32 tier below 1e-4, from the geometry strings to
+    converged energies.  Checked per molecule against the batched FP64
+    core-guess run (|dE| <= 1 uEh), which is checked against the plain FP64
+    path one molecule at a time (Molecule(g).RHF(direct=True)) on every 8th
+    geometry; the plain path's rate is measured on those."""
     import numpy as np
-    from mmd.batch import run_batch_device
+    from mmd.batch import run_batch_device, _device_contexts
     from mmd.molecule import Molecule
     geoms = batch_geometries()
     G = args.group
-    full_kw = dict(group=G, guess="learned", fp32_below=1e-4)
+    full_kw = dict(group=G, guess="learned", fp32_below=1e-4, streams=args.batch_streams)
     for _ in range(max(1, args.warmup)):
-        run_batch_device(geoms[:2 * G], "6-31g", **full_kw)
-        run_batch_device(geoms[:G], "6-31g", group=G)
+        run_batch_device(geoms[:4 * G], "6-31g", **full_kw)
+        run_batch_device(geoms[:2 * G], "6-31g", group=G, streams=args.batch_streams)
     print("warmup done", file=sys.stderr, flush=True)
+    ctxs = _device_contexts(0, max(1, args.batch_streams))
+    step_s = []
+    full = None
+    per = None
     t0 = time.perf_counter()
     for s in range(args.steps):
+        prof = (not args.no_profile) and s == args.steps - 1
+        if prof:
+            for c in ctxs:
+                c.set_profiling(True)
+        ts = time.perf_counter()
         full = run_batch_device(geoms, "6-31g", **full_kw)
-        print("full-form step %d done" % s, file=sys.stderr, flush=True)
+        step_s.append(time.perf_counter() - ts)
+        if prof:
+            per = _profile_sums(ctxs)
+            for c in ctxs:
+                c.set_profiling(False)
+        print("full-form step %d done (%.3f s%s)" % (s, step_s[-1], ", profiled" if prof else ""),
+              file=sys.stderr, flush=True)
     el_full = (time.perf_counter() - t0) / args.steps
     t1 = time.perf_counter()
-    ref = run_batch_device(geoms, "6-31g", group=G)
+    ref = run_batch_device(geoms, "6-31g", group=G, streams=args.batch_streams)
     el_64 = time.perf_counter() - t1
     print("batched FP64 done", file=sys.stderr, flush=True)
     e_full = np.array([r["energy"] for r in full])
@@ -310,13 +523,21 @@ def bench_batch_device(args):
         d_plain = max(d_plain, abs(m.energy.real - e_64[i]))
     plain_rate = len(sample) / (time.perf_counter() - t2)
     print("plain one-at-a-time done", file=sys.stderr, flush=True)
-    value = len(geoms) / el_full if dE.max() <= 1e-6 and all(r["converged"] for r in full) else 0.0
-    return {
-        "value": value, "ms_per_step": 1e3 * el_full,
-        "config": {"workload": "512 x (H2O)n n=2..6 6-31G direct RHF, full form: batched on the device "
-                               "(groups of %d, HIP graph) + learned start + FP32 tier below 1e-4" % G,
-                   "seed": 1, "basis": "6-31g", "molecules": len(geoms), "group": G, "conver": 1e-8,
-                   "acc2e": 1e-12},
+    ok = dE.max() <= 1e-6 and all(r["converged"] for r in full) and d_plain <= 1e-6
+    value = len(geoms) / el_full if ok else 0.0
+    ns = [len([ln for ln in g.split("\n")[1:] if ln.strip()]) // 3 for g in geoms]
+    out = {
+        "metric": METRIC, "value": value, "unit": "SCF/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * el_full, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
+        "config": {"workload": "512 x (H2O)n, n=2..6, 6-31G direct RHF, full form (batched on the device in "
+                               "groups of %d, %d groups in flight, HIP-graph iterations, learned start, FP32 "
+                               "tier below 1e-4) vs plain FP64 one at a time" % (G, args.batch_streams),
+                   "molecules": len(geoms), "seed": 1, "basis": "6-31g", "group": G,
+                   "groups_in_flight": args.batch_streams, "conver": 1e-8, "acc2e": 1e-12,
+                   "size_mix": {str(k): ns.count(k) for k in sorted(set(ns))}, "hw_queues": int(HW_QUEUES)},
+        "step": "one pass over the 512 molecules, geometry strings to converged energies",
+        "step_s": step_s,
         "max_abs_dE_full_vs_fp64_uEh": float(dE.max() * 1e6),
         "all_converged": bool(all(r["converged"] for r in full)),
         "mean_iterations_full": float(np.mean([r["iterations"] for r in full])),
@@ -327,6 +548,20 @@ def bench_batch_device(args):
         "max_abs_dE_plain_vs_batched_fp64": float(d_plain),
         "speedup_vs_plain_fp64": (len(geoms) / el_full) / plain_rate,
     }
+    if per:
+        out["roofline"] = headline_roofline(per)
+        nb = {k: max(1, v[6]) for k, v in per.items()}
+        out["class_breakdown"] = {
+            "(%d%d|%d%d)" % k: {"fp64_ms_per_launch": v[4] / nb[k], "fp32_ms_per_launch": v[5] / nb[k],
+                                "shell_quartets_per_launch": (v[0] + v[1]) / nb[k],
+                                "prim_quartets_per_launch": (v[2] + v[3]) / nb[k], "launches": v[6]}
+            for k, v in sorted(per.items(), key=lambda kv: -kv[1][4])}
+    out["cfg2"] = cfg2_extras(args)
+    print("cfg2 extras done", file=sys.stderr, flush=True)
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_batch(geoms, args.cpu_seconds)
+        print("cpu baseline done", file=sys.stderr, flush=True)
+    return out
 
 
 def bench_ao2mo(args):
@@ -385,10 +620,12 @@ def main():
     ap.add_argument("--serial-classes", action="store_true",
                     help="run the Fock classes one after the other in the profiled step (for counter runs)")
     ap.add_argument("--group", type=int, default=64, help="molecules per block-diagonal batch (batch workloads)")
-    ap.add_argument("--workload", choices=("h2o8", "h2o16-tier", "batch512", "batch-device", "ao2mo",
-                                           "copy-calibration"), default="h2o8",
-                    help="h2o8: (H2O)8 cc-pVDZ FP64 (default); h2o16-tier: (H2O)16 cc-pVDZ FP64 vs FP32 tier; "
-                         "batch512: 512 small 6-31G clusters")
+    ap.add_argument("--batch-streams", type=int, default=2, help="groups in flight at once (headline)")
+    ap.add_argument("--workload", choices=("headline", "h2o8", "h2o16-tier", "batch512", "ao2mo",
+                                           "copy-calibration"), default="headline",
+                    help="headline: 512 x (H2O)2-6 6-31G, full form vs plain FP64 (default); "
+                         "h2o8: (H2O)8 cc-pVDZ FP64; h2o16-tier: (H2O)16 cc-pVDZ FP64 vs FP32 tier; "
+                         "batch512: 512 small 6-31G clusters through worker processes")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -407,8 +644,8 @@ def main():
     if _native.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
-    if args.workload == "batch-device":
-        print(json.dumps(bench_batch_device(args)))
+    if args.workload == "headline":
+        print(json.dumps(bench_headline(args)))
         return
     if args.workload == "h2o16-tier":
         print(json.dumps(bench_tier(args)))

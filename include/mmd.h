@@ -178,6 +178,16 @@ int mmd_boys(mmd_ctx *ctx, int n, const double *m, const double *T, int table, d
  */
 int mmd_set_profiling(mmd_ctx *ctx, int on);
 int mmd_fock_stats(mmd_ctx *ctx, int32_t *lclass, int64_t *nquart, int64_t *nprim, double *ms);
+/*
+ * Sums over every profiled build since profiling was last turned on
+ * (symmetrised builds of mmd_fock_delta and the builds of mmd_batch_rhf,
+ * whose iterations are then launched one by one instead of from a graph),
+ * per class and tier: nquart, nprim and ms get 2 entries per class (FP64
+ * launches, FP32-tier launches), builds 1 (builds in which the class ran).
+ * Returns the number of classes (21).
+ */
+int mmd_fock_profile_sums(mmd_ctx *ctx, int32_t *lclass, int64_t *nquart, int64_t *nprim, double *ms,
+                          int64_t *builds);
 
 /* Device time (ms) of the last mmd_fock_delta call, from HIP events. */
 int mmd_last_fock_ms(mmd_ctx *ctx, double *ms);

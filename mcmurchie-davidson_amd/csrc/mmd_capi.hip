@@ -89,6 +89,14 @@ struct DevBuf {
 // the first context on it is created (scratch_for_device).
 // ---------------------------------------------------------------------------
 constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(16) << 20;   // 16 MiB
+constexpr int NPRIM_SLOTS = 4 * NQCLASS;   // survivor counters of one Fock build
+
+// Per-class sums over profiled builds; index 0 the FP64 launches, 1 the FP32 tier.
+struct ProfileSum {
+    int64_t nquart[2] = {0, 0}, nprim[2] = {0, 0};
+    double ms[2] = {0.0, 0.0};
+    int64_t builds = 0;
+};
 static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
 
 namespace {
@@ -191,14 +199,14 @@ struct mmd_ctx {
 
     // per-class statistics of the last symmetrised Fock build
     int profiling = 0;                  // 0 off; 1 events, classes concurrent; 2 events, classes serial
-    hipEvent_t evc[2 * NQCLASS] = {};
+    hipEvent_t evc[3 * NQCLASS] = {};   // per class: start, after the FP64 launch, end
     // Stream plan of the Fock build: class durations measured with the
     // per-class events in one build out of PLAN_EVERY (0: not measured yet).
     static constexpr int PLAN_EVERY = 16;
     double plan_ms[NQCLASS] = {0};
     int plan_countdown = 0;
     bool plan_measuring = false;
-    DevBuf<unsigned long long> d_nprim;   // [0,21): primitive quartets, [21,42): shell quartets
+    DevBuf<unsigned long long> d_nprim;   // NPRIM_SLOTS counters (FockScratch::nprim)
     DevBuf<int> d_rowctr;
     // Work units of the row-kernel classes (built once per pair table):
     // class qc owns units [unit_off[qc], unit_off[qc+1]) with ket segment
@@ -229,6 +237,7 @@ struct mmd_ctx {
     std::vector<double> s_B;         // 8 x 8 error overlaps, window order
     int64_t st_nprim[NQCLASS] = {0};
     double st_ms[NQCLASS] = {0};
+    ProfileSum psum[NQCLASS];           // sums over the builds since profiling was turned on
 };
 
 namespace {
@@ -582,12 +591,15 @@ ClassArgs base_args(mmd_ctx* c)
 struct FockScratch {
     int* rowctr = nullptr;                 // 2 * NQCLASS
     int* count = nullptr;                  // 2 * NQCLASS: survivor-list lengths (FP64 lists, then FP32)
-    unsigned long long* nprim = nullptr;   // 3 * NQCLASS
+    // NPRIM_SLOTS counters: primitive quartets (FP64 tier), shell quartets of
+    // the row kernels (FP64, FP32 tier), primitive quartets (FP32 tier)
+    unsigned long long* nprim = nullptr;
     double* Dsh = nullptr;                 // nshell^2
     int2* surv = nullptr;
     int64_t surv_off[NQCLASS] = {0};       // survivor lists, packed: wave classes only
     int64_t surv_cap = -1;                 // entries available at surv; < 0: grow the context's buffer
     bool allow_measure = true;             // may create events and re-measure the stream plan
+    bool profile_events = false;           // with profiling on, time the classes even without allow_measure
 };
 
 // Launch the Fock kernels of every class.  Classes with a row kernel screen
@@ -633,7 +645,7 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol, FockScratch& W0)
     }
     HIP_TRY(launch_fill_int(2 * NQCLASS, W0.rowctr, 0, c->st));
     HIP_TRY(launch_fill_int(2 * NQCLASS, W0.count, 0, c->st));
-    HIP_TRY(launch_fill_int(6 * NQCLASS, reinterpret_cast<int*>(W0.nprim), 0, c->st));
+    HIP_TRY(launch_fill_int(2 * NPRIM_SLOTS, reinterpret_cast<int*>(W0.nprim), 0, c->st));
     // Classes run concurrently on the main and side streams (longest first,
     // each to the least loaded stream), so that the small classes fill the
     // GPU beside the large ones.  Profiling mode 1 brackets every class with
@@ -665,8 +677,8 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol, FockScratch& W0)
     for (int oi = 0; oi < nord; ++oi) measured = measured && c->plan_ms[order[oi]] > 0.0;
     if (measured)
         for (int oi = 0; oi < nord; ++oi) cost[order[oi]] = c->plan_ms[order[oi]];
-    if (measure && !c->evc[0])
-        for (int i = 0; i < 2 * NQCLASS; ++i) HIP_TRY(hipEventCreate(&c->evc[i]));
+    if ((measure || (c->profiling && W0.profile_events)) && !c->evc[0])
+        for (int i = 0; i < 3 * NQCLASS; ++i) HIP_TRY(hipEventCreate(&c->evc[i]));
     c->plan_measuring = measure;
     std::sort(order, order + nord, [&](int x, int y) { return cost[x] > cost[y]; });
     hipStream_t streams[1 + mmd_ctx::NSIDE] = {c->st};
@@ -690,8 +702,8 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol, FockScratch& W0)
         qclass_decode(qc, &la, &lb, &lc, &ld);
         const int nb = c->pc_off[pcb + 1] - c->pc_off[pcb];
         const int nk = c->pc_off[pck + 1] - c->pc_off[pck];
-        const bool events = measure || (c->profiling && W0.allow_measure);
-        if (events) HIP_TRY(hipEventRecord(c->evc[2 * qc], sq));
+        const bool events = measure || (c->profiling && (W0.allow_measure || W0.profile_events));
+        if (events) HIP_TRY(hipEventRecord(c->evc[3 * qc], sq));
         if (row_kernel[qc]) {
             RowArgs R;
             R.boff = c->pc_off[pcb];
@@ -710,7 +722,10 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol, FockScratch& W0)
             const int64_t units = c->unit_off[qc + 1] - c->unit_off[qc];
             R.nunits = (int)units;
             if (units == 0) {
-                if (events) HIP_TRY(hipEventRecord(c->evc[2 * qc + 1], sq));
+                if (events) {
+                    HIP_TRY(hipEventRecord(c->evc[3 * qc + 1], sq));
+                    HIP_TRY(hipEventRecord(c->evc[3 * qc + 2], sq));
+                }
                 continue;
             }
             R.stat_nquart = W0.nprim + NQCLASS + qc;
@@ -719,12 +734,14 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol, FockScratch& W0)
             R.tier_hi = HUGE_VAL;
             const int grid = (int)std::min<int64_t>(units, 2048);
             HIP_TRY(launch_fock_class(qc, 0, A, R, grid, sq));
+            if (events) HIP_TRY(hipEventRecord(c->evc[3 * qc + 1], sq));
             if (tier32) {
                 // FP32 tier: the same units, quartets with tol <= bound < fp32_below
                 R.tier_lo = 0.0;
                 R.tier_hi = c->fp32_below;
                 R.row_counter = W0.rowctr + NQCLASS + qc;
                 R.stat_nquart = W0.nprim + 2 * NQCLASS + qc;
+                R.stat_nprim = c->profiling ? W0.nprim + 3 * NQCLASS + qc : nullptr;
                 HIP_TRY(launch_fock_class(qc, 1, A, R, grid, sq));
             }
         } else {
@@ -741,15 +758,16 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol, FockScratch& W0)
             if (c->profiling)
                 HIP_TRY(launch_count_prims(W.quart, W.count, c->d_pairs.p, W0.nprim + qc, sq));
             HIP_TRY(launch_fock_wave_class(la, lb, lc, ld, 0, W, c->qc_total[qc], sq));
+            if (events) HIP_TRY(hipEventRecord(c->evc[3 * qc + 1], sq));
             if (tier32) {
                 W.quart = surv32;
                 W.count = W0.count + NQCLASS + qc;
                 if (c->profiling)
-                    HIP_TRY(launch_count_prims(W.quart, W.count, c->d_pairs.p, W0.nprim + qc, sq));
+                    HIP_TRY(launch_count_prims(W.quart, W.count, c->d_pairs.p, W0.nprim + 3 * NQCLASS + qc, sq));
                 HIP_TRY(launch_fock_wave_class(la, lb, lc, ld, 1, W, c->qc_total[qc], sq));
             }
         }
-        if (events) HIP_TRY(hipEventRecord(c->evc[2 * qc + 1], sq));
+        if (events) HIP_TRY(hipEventRecord(c->evc[3 * qc + 2], sq));
     }
     for (int i = 1; i < nstream; ++i) {
         if (!used[i]) continue;
@@ -760,24 +778,38 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol, FockScratch& W0)
     return MMD_OK;
 }
 
-// Per-class statistics and event times of the last build (profiling on).
-// Called after the build's end event, so the readback is not timed.
+// Per-class statistics and event times of the last build (profiling on),
+// also added to the sums since profiling was turned on.  Called after the
+// build's end event, so the readback is not timed.
 int collect_profile(mmd_ctx* c)
 {
     int h[2 * NQCLASS];
-    unsigned long long hp[3 * NQCLASS];
+    unsigned long long hp[NPRIM_SLOTS];
     HIP_TRY(hipMemcpyAsync(h, c->d_count.p, sizeof(int) * 2 * NQCLASS, hipMemcpyDeviceToHost, c->st));
     HIP_TRY(hipMemcpyAsync(hp, c->d_nprim.p, sizeof(hp), hipMemcpyDeviceToHost, c->st));
     HIP_TRY(hipStreamSynchronize(c->st));
     for (int qc = 0; qc < NQCLASS; ++qc) {
-        c->st_nquart[qc] = c->qc_total[qc]
-                               ? (c->last_row_kernel[qc] ? (int64_t)(hp[NQCLASS + qc] + hp[2 * NQCLASS + qc])
-                                                         : (int64_t)h[qc] + h[NQCLASS + qc])
-                               : 0;
-        c->st_nprim[qc] = c->qc_total[qc] ? (int64_t)hp[qc] : 0;
-        float ms = 0.f;
-        if (c->qc_total[qc]) HIP_TRY(hipEventElapsedTime(&ms, c->evc[2 * qc], c->evc[2 * qc + 1]));
+        if (!c->qc_total[qc]) {
+            c->st_nquart[qc] = c->st_nprim[qc] = 0;
+            c->st_ms[qc] = 0.0;
+            continue;
+        }
+        const int64_t q64 = c->last_row_kernel[qc] ? (int64_t)hp[NQCLASS + qc] : (int64_t)h[qc];
+        const int64_t q32 = c->last_row_kernel[qc] ? (int64_t)hp[2 * NQCLASS + qc] : (int64_t)h[NQCLASS + qc];
+        c->st_nquart[qc] = q64 + q32;
+        c->st_nprim[qc] = (int64_t)(hp[qc] + hp[3 * NQCLASS + qc]);
+        float ms = 0.f, ms64 = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, c->evc[3 * qc], c->evc[3 * qc + 2]));
+        HIP_TRY(hipEventElapsedTime(&ms64, c->evc[3 * qc], c->evc[3 * qc + 1]));
         c->st_ms[qc] = ms;
+        ProfileSum& a = c->psum[qc];
+        a.nquart[0] += q64;
+        a.nquart[1] += q32;
+        a.nprim[0] += (int64_t)hp[qc];
+        a.nprim[1] += (int64_t)hp[3 * NQCLASS + qc];
+        a.ms[0] += ms64;
+        a.ms[1] += ms - ms64;
+        a.builds += 1;
     }
     return MMD_OK;
 }
@@ -785,10 +817,9 @@ int collect_profile(mmd_ctx* c)
 }  // namespace
 
 namespace {
-// Install a Schwarz diagonal (N*N, (pq|pq)) and rebuild the sorted shell-pair
-// tables with their factors q = max sqrt((ab|ab)).
-// The Schwarz matrix (pq|pq) is in d_schwarz; diag views it on the host for
-// the pair factors.
+// The Schwarz matrix (pq|pq) is in d_schwarz: its square roots for the
+// screen, and the sorted shell-pair tables with their factors
+// q = max sqrt((ab|ab)) from the host view diag.
 int finish_schwarz(mmd_ctx* ctx, const DiagView& diag)
 {
     const size_t nn = (size_t)ctx->nbf * ctx->nbf;
@@ -803,6 +834,7 @@ int finish_schwarz(mmd_ctx* ctx, const DiagView& diag)
     return build_units(ctx);
 }
 
+// Install a Schwarz diagonal (N*N, (pq|pq), host) and rebuild the pair tables.
 int set_schwarz(mmd_ctx* ctx, const std::vector<double>& h)
 {
     const int N = ctx->nbf;
@@ -1231,7 +1263,7 @@ int mmd_fock_delta(mmd_ctx* ctx, const double* dP_re, const double* dP_im, doubl
         A.G = ctx->d_G.p;
         HIP_TRY(launch_fill(nn, ctx->d_G.p, 0.0, ctx->st));
         HIP_TRY(ctx->d_rowctr.reserve(2 * NQCLASS));
-        HIP_TRY(ctx->d_nprim.reserve(3 * NQCLASS));
+        HIP_TRY(ctx->d_nprim.reserve(NPRIM_SLOTS));
         FockScratch W;
         W.rowctr = ctx->d_rowctr.p;
         W.count = ctx->d_count.p;
@@ -1276,7 +1308,7 @@ int mmd_fock_delta(mmd_ctx* ctx, const double* dP_re, const double* dP_im, doubl
     if (!raw && ctx->plan_measuring) {
         for (int qc = 0; qc < NQCLASS; ++qc) {
             float cms = 0.f;
-            if (ctx->qc_total[qc]) HIP_TRY(hipEventElapsedTime(&cms, ctx->evc[2 * qc], ctx->evc[2 * qc + 1]));
+            if (ctx->qc_total[qc]) HIP_TRY(hipEventElapsedTime(&cms, ctx->evc[3 * qc], ctx->evc[3 * qc + 2]));
             ctx->plan_ms[qc] = cms > 0.f ? cms : 1e-3;
         }
         ctx->plan_measuring = false;
@@ -1319,7 +1351,7 @@ int mmd_fock_delta_noalloc(mmd_ctx* ctx, const double* dP, double tol, double* G
     int* count = reinterpret_cast<int*>(base + off);
     off += up(sizeof(int) * 2 * NQCLASS);
     unsigned long long* nprim = reinterpret_cast<unsigned long long*>(base + off);
-    off += up(sizeof(unsigned long long) * 3 * NQCLASS);
+    off += up(sizeof(unsigned long long) * NPRIM_SLOTS);
     if (off > NOALLOC_SCRATCH_BYTES)
         return fail(MMD_ERR_ARG, "a basis of %d functions needs %zu bytes of scratch; the limit is %zu", N, off,
                     NOALLOC_SCRATCH_BYTES);
@@ -1421,7 +1453,9 @@ int mmd_set_profiling(mmd_ctx* ctx, int on)
     int rc = check_ctx(ctx);
     if (rc) return rc;
     if (on && !ctx->evc[0])
-        for (int i = 0; i < 2 * NQCLASS; ++i) HIP_TRY(hipEventCreate(&ctx->evc[i]));
+        for (int i = 0; i < 3 * NQCLASS; ++i) HIP_TRY(hipEventCreate(&ctx->evc[i]));
+    if (on)
+        for (auto& a : ctx->psum) a = ProfileSum();
     if (on < 0 || on > 2) return fail(MMD_ERR_ARG, "profiling mode must be 0, 1 or 2");
     ctx->profiling = on;
     return MMD_OK;
@@ -1435,6 +1469,23 @@ int mmd_fock_stats(mmd_ctx* ctx, int32_t* lclass, int64_t* nquart, int64_t* npri
         if (nquart) nquart[qc] = ctx->st_nquart[qc];
         if (nprim) nprim[qc] = ctx->st_nprim[qc];
         if (ms) ms[qc] = ctx->st_ms[qc];
+    }
+    return NQCLASS;
+}
+
+int mmd_fock_profile_sums(mmd_ctx* ctx, int32_t* lclass, int64_t* nquart, int64_t* nprim, double* ms,
+                          int64_t* builds)
+{
+    if (!ctx) return fail(MMD_ERR_ARG, "null context");
+    for (int qc = 0; qc < NQCLASS; ++qc) {
+        const ProfileSum& a = ctx->psum[qc];
+        if (lclass) qclass_decode(qc, &lclass[4 * qc], &lclass[4 * qc + 1], &lclass[4 * qc + 2], &lclass[4 * qc + 3]);
+        for (int t = 0; t < 2; ++t) {
+            if (nquart) nquart[2 * qc + t] = a.nquart[t];
+            if (nprim) nprim[2 * qc + t] = a.nprim[t];
+            if (ms) ms[2 * qc + t] = a.ms[t];
+        }
+        if (builds) builds[qc] = a.builds;
     }
     return NQCLASS;
 }
@@ -1547,6 +1598,7 @@ int enqueue_fock_sym(mmd_ctx* c, const double* dP, double tol, double* Gout)
     W.rowctr = c->d_rowctr.p;
     W.count = c->d_count.p;
     W.nprim = c->d_nprim.p;
+    W.profile_events = true;
     W.Dsh = c->d_Dsh.p;
     W.surv = c->d_surv.p;
     W.surv_cap = (int64_t)c->d_surv.n;   // reserved by the caller
@@ -1646,7 +1698,7 @@ int mmd_batch_rhf(mmd_ctx* ctx, const int32_t* nocc, const double* enuc, const d
     }
     // buffers of the Fock build, reserved before any capture
     HIP_TRY(c->d_rowctr.reserve(2 * NQCLASS));
-    HIP_TRY(c->d_nprim.reserve(3 * NQCLASS));
+    HIP_TRY(c->d_nprim.reserve(NPRIM_SLOTS));
     {
         int64_t wave = 0;
         for (int qc = 0; qc < NQCLASS; ++qc) {
@@ -1714,7 +1766,10 @@ int mmd_batch_rhf(mmd_ctx* ctx, const int32_t* nocc, const double* enuc, const d
         HIP_TRY(launch_batch_step(B, 1, diis, conver, st));
         return MMD_OK;
     };
-    if (use_graph) {
+    // profiling: per-class events and counters read after every build, so
+    // the iterations are launched one by one
+    const bool profiled = c->profiling != 0;
+    if (use_graph && !profiled) {
         if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) == hipSuccess) {
             const int r = iteration();
             hipGraph_t g = nullptr;
@@ -1737,6 +1792,7 @@ int mmd_batch_rhf(mmd_ctx* ctx, const int32_t* nocc, const double* enuc, const d
         } else if ((rc = iteration())) {
             return rc;
         }
+        if (profiled && (rc = collect_profile(c))) return rc;
         if (it % 4 == 0 || it == maxiter - 1) {
             HIP_TRY(launch_batch_count(B, c->b_count.p, st));
             HIP_TRY(hipMemcpyAsync(&left, c->b_count.p, sizeof(int), hipMemcpyDeviceToHost, st));

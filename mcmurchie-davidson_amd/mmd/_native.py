@@ -84,6 +84,8 @@ lib.mmd_set_profiling.restype = _c_int
 lib.mmd_set_profiling.argtypes = [_ctx_p, _c_int]
 lib.mmd_fock_stats.restype = _c_int
 lib.mmd_fock_stats.argtypes = [_ctx_p, _vp, _vp, _vp, _vp]
+lib.mmd_fock_profile_sums.restype = _c_int
+lib.mmd_fock_profile_sums.argtypes = [_ctx_p, _vp, _vp, _vp, _vp, _vp]
 lib.mmd_last_fock_ms.restype = _c_int
 lib.mmd_last_fock_ms.argtypes = [_ctx_p, ctypes.POINTER(_c_double)]
 lib.mmd_set_precision.restype = _c_int
@@ -123,7 +125,7 @@ EXPORTED = [
     "mmd_last_error", "mmd_version", "mmd_device_count", "mmd_ctx_create", "mmd_ctx_destroy",
     "mmd_set_basis", "mmd_set_atoms", "mmd_one_electron", "mmd_schwarz", "mmd_set_schwarz",
     "mmd_eri_tensor", "mmd_eri_list", "mmd_fock_delta", "mmd_fock_delta_noalloc", "mmd_jk_tensor", "mmd_boys",
-    "mmd_last_fock_ms", "mmd_set_profiling", "mmd_fock_stats", "mmd_set_precision", "mmd_fock_tiers",
+    "mmd_last_fock_ms", "mmd_set_profiling", "mmd_fock_stats", "mmd_fock_profile_sums", "mmd_set_precision", "mmd_fock_tiers",
     "mmd_set_streams", "mmd_set_lds_limit", "mmd_overlap_powers", "mmd_dgemm", "mmd_set_molecules",
     "mmd_batch_rhf", "mmd_scf_begin", "mmd_scf_step", "mmd_scf_density", "mmd_scf_get", "mmd_ao2mo",
     "mmd_mlp_create", "mmd_mlp_destroy", "mmd_mlp_nparam", "mmd_mlp_set_params", "mmd_mlp_get_params",
@@ -346,6 +348,21 @@ class Context:
         if n < 0:
             check(n)
         return lc.reshape(21, 4)[:n], nq[:n], npr[:n], ms[:n]
+
+    def fock_profile_sums(self):
+        """Sums since profiling was turned on, per class: (l_a, l_b, l_c, l_d),
+        shell quartets, primitive quartets and kernel ms, each as (FP64
+        launches, FP32-tier launches), and the number of builds."""
+        lc = np.zeros(84, dtype=np.int32)
+        nq = np.zeros(42, dtype=np.int64)
+        npr = np.zeros(42, dtype=np.int64)
+        ms = np.zeros(42)
+        nb = np.zeros(21, dtype=np.int64)
+        n = lib.mmd_fock_profile_sums(self._h, _ptr(lc), _ptr(nq), _ptr(npr), _ptr(ms), _ptr(nb))
+        if n < 0:
+            check(n)
+        return (lc.reshape(21, 4)[:n], nq.reshape(21, 2)[:n], npr.reshape(21, 2)[:n], ms.reshape(21, 2)[:n],
+                nb[:n])
 
     def last_fock_ms(self):
         v = _c_double(0.0)

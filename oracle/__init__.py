@@ -66,6 +66,8 @@ def lib():
         L.orc_do_eris.argtypes = bfs + [_i64, _pd]
         L.orc_schwarz.restype = None
         L.orc_schwarz.argtypes = bfs + [_i64, _pd]
+        L.orc_form_pt_survivors.restype = _i64
+        L.orc_form_pt_survivors.argtypes = [_i64, _pd, _pd, _d]
         L.orc_form_pt.restype = _i64
         L.orc_form_pt.argtypes = bfs + [_i64, _pd, ctypes.c_void_p, _pd, _d, _pd,
                                         ctypes.c_void_p, _i64, _i64]
@@ -259,6 +261,14 @@ class OracleMolecule:
         if cplx:
             G = G + 1j * Gi
         return G, nsurv
+
+
+def form_pt_survivors(dP, screen, tol):
+    """Quartets formPT keeps for the real difference dP = P - P_old
+    (cython/fock.pyx:38-57), counted without computing integrals."""
+    dP = np.ascontiguousarray(np.real(dP), dtype=np.float64)
+    return int(lib().orc_form_pt_survivors(dP.shape[0], dP, np.ascontiguousarray(screen, dtype=np.float64),
+                                           float(tol)))
 
 
 def fock_elements(mol, P, S_diag, tol, pq):

@@ -345,6 +345,36 @@ int64_t orc_form_pt(ORC_BFS_ARGS, int64_t N, const double *dPr, const double *dP
 }
 
 
+/* The number of quartets formPT's screen keeps for a real dP, without
+ * computing any integral: the loop and test of orc_form_pt
+ * (cython/fock.pyx:38-57).  Used to state the survivors of each iteration
+ * next to the timed formPT samples of the CPU baseline. */
+int64_t orc_form_pt_survivors(int64_t N, const double *dPr, const double *screen, double tol)
+{
+    int64_t survivors = 0;
+#define M(A, a, b) ((A)[(a) * N + (b)])
+    for (int64_t i = 0; i < N; ++i)
+        for (int64_t j = 0; j <= i; ++j) {
+            const int64_t ij = i * (i + 1) / 2 + j;
+            const double sij = sqrt(screen[ij]);
+            const double pij = 4.0 * fabs(M(dPr, i, j));
+            for (int64_t k = 0; k <= i; ++k)
+                for (int64_t l = 0; l <= k; ++l) {
+                    const int64_t kl = k * (k + 1) / 2 + l;
+                    if (kl > ij) break;
+                    double dm = pij, x;
+                    x = 4.0 * fabs(M(dPr, k, l)); if (x > dm) dm = x;
+                    x = fabs(M(dPr, i, k)); if (x > dm) dm = x;
+                    x = fabs(M(dPr, i, l)); if (x > dm) dm = x;
+                    x = fabs(M(dPr, j, k)); if (x > dm) dm = x;
+                    x = fabs(M(dPr, j, l)); if (x > dm) dm = x;
+                    if (sij * sqrt(screen[kl]) * dm >= tol) ++survivors;
+                }
+        }
+#undef M
+    return survivors;
+}
+
 /* Selected elements of the symmetrised direct Fock contribution
  * 0.5 (G + G^T) with G = formPT(P, 0) (cython/fock.pyx:38-85 followed by
  * mmd/scf.py:93) for a real symmetric P, one element at a time:

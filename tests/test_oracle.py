@@ -345,6 +345,26 @@ def test_fock_elements_equal_formpt(oracle):
     assert nsurv < N * (N + 1) // 2 * (N * (N + 1) // 2 + 1) // 2
 
 
+def test_form_pt_survivor_count(oracle):
+    """The integral-free survivor count (used by the CPU baseline) equals the
+    number of quartets orc_form_pt keeps, at thresholds that keep all, some
+    and none; with tol = 0 it is every canonical quartet."""
+    om = oracle.OracleMolecule(WATER, _gbs("6-31g"))
+    N = om.nbasis
+    rng = np.random.default_rng(8)
+    A = rng.standard_normal((N, N))
+    dP = (A + A.T) * 1e-4
+    scr = om.schwarz()
+    npair = N * (N + 1) // 2
+    assert oracle.form_pt_survivors(dP, scr, 0.0) == npair * (npair + 1) // 2
+    counts = []
+    for tol in (1e-6, 1e-5, 1e-4, 1e3):
+        _, nsurv = om.form_pt(dP.astype(complex), np.zeros((N, N), complex), scr, tol)
+        assert oracle.form_pt_survivors(dP, scr, tol) == nsurv
+        counts.append(nsurv)
+    assert npair * (npair + 1) // 2 > counts[0] > counts[1] > counts[2] > counts[3] == 0
+
+
 def test_rhf_water_sto3g_reference(oracle):
     """tests/test002.py (E = -74.942079928192, atol 1e-12) and the README
     dipole 1.53400931 D, at the geometry those values belong to."""

@@ -478,9 +478,9 @@ def bench_headline(args):
     import numpy as np
     from mmd.batch import run_batch_device, _device_contexts
     from mmd.molecule import Molecule
-    geoms = batch_geometries()
+    geoms = batch_geometries(args.molecules)
     G = args.group
-    full_kw = dict(group=G, guess="learned", fp32_below=1e-4, streams=args.batch_streams)
+    full_kw = dict(group=G, guess="learned", fp32_below=args.fp32_below, streams=args.batch_streams)
     for _ in range(max(1, args.warmup)):
         run_batch_device(geoms[:4 * G], "6-31g", **full_kw)
         run_batch_device(geoms[:2 * G], "6-31g", group=G, streams=args.batch_streams)
@@ -530,13 +530,14 @@ def bench_headline(args):
         "metric": METRIC, "value": value, "unit": "SCF/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * el_full, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
-        "config": {"workload": "512 x (H2O)n, n=2..6, 6-31G direct RHF, full form (batched on the device in "
+        "config": {"workload": "%d x (H2O)n, n=2..6, 6-31G direct RHF, full form (batched on the device in "
                                "groups of %d, %d groups in flight, HIP-graph iterations, learned start, FP32 "
-                               "tier below 1e-4) vs plain FP64 one at a time" % (G, args.batch_streams),
+                               "tier below %g) vs plain FP64 one at a time" % (len(geoms), G, args.batch_streams, args.fp32_below),
                    "molecules": len(geoms), "seed": 1, "basis": "6-31g", "group": G,
-                   "groups_in_flight": args.batch_streams, "conver": 1e-8, "acc2e": 1e-12,
+                   "groups_in_flight": args.batch_streams, "fp32_below": args.fp32_below, "conver": 1e-8,
+                   "acc2e": 1e-12,
                    "size_mix": {str(k): ns.count(k) for k in sorted(set(ns))}, "hw_queues": int(HW_QUEUES)},
-        "step": "one pass over the 512 molecules, geometry strings to converged energies",
+        "step": "one pass over the %d molecules, geometry strings to converged energies" % len(geoms),
         "step_s": step_s,
         "max_abs_dE_full_vs_fp64_uEh": float(dE.max() * 1e6),
         "all_converged": bool(all(r["converged"] for r in full)),
@@ -609,6 +610,35 @@ def copy_calibration():
             "launches": 5, "GB_per_s": 2 * n * 5 / el / 1e9}
 
 
+def pmc_summary(paths):
+    """Per-kernel averages per dispatch of rocprofv3 --pmc counter_collection
+    CSV files (one line per kernel, most GRBM_GUI_ACTIVE first): what the
+    committed counter profiles are made from."""
+    import csv
+    import collections
+    agg = collections.defaultdict(lambda: collections.defaultdict(float))
+    disp = collections.defaultdict(lambda: collections.defaultdict(set))
+    meta = {}
+    for path in paths:
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                k = r["Kernel_Name"]
+                agg[k][r["Counter_Name"]] += float(r["Counter_Value"])
+                disp[k][r["Counter_Name"]].add(r["Dispatch_Id"])
+                meta[k] = ",".join(r.get(x, "") for x in ("VGPR_Count", "Accum_VGPR_Count", "SGPR_Count",
+                                                          "LDS_Block_Size", "Workgroup_Size", "Grid_Size",
+                                                          "Scratch_Size"))
+    cols = sorted({c for v in agg.values() for c in v})
+    lines = ["kernel|dispatches|" + "|".join(cols) + "|vgpr,agpr,sgpr,lds,wg,grid,scratch"]
+    key = "GRBM_GUI_ACTIVE" if "GRBM_GUI_ACTIVE" in cols else cols[0]
+    for k, v in sorted(agg.items(), key=lambda kv: -kv[1].get(key, 0)):
+        n = max(len(d) for d in disp[k].values())
+        vals = "|".join("%.4g" % (v[c] / max(1, len(disp[k][c]))) if c in v else "" for c in cols)
+        lines.append("%s|%d|%s|%s" % (k.replace("mmd::", "").replace("(anonymous namespace)::", "")[:90], n, vals,
+                                      meta[k]))
+    return "\n".join(lines)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -620,13 +650,23 @@ def main():
     ap.add_argument("--serial-classes", action="store_true",
                     help="run the Fock classes one after the other in the profiled step (for counter runs)")
     ap.add_argument("--group", type=int, default=64, help="molecules per block-diagonal batch (batch workloads)")
-    ap.add_argument("--batch-streams", type=int, default=2, help="groups in flight at once (headline)")
+    ap.add_argument("--batch-streams", type=int, default=3, help="groups in flight at once (headline)")
+    ap.add_argument("--molecules", type=int, default=512,
+                    help="headline molecules (512 is the workload; fewer only for counter runs)")
+    ap.add_argument("--fp32-below", type=float, default=0.0,
+                    help="FP32 tier threshold of the headline's full form (0: all FP64; at these sizes the "
+                         "tier's second pass over the units costs more than its FP32 arithmetic saves)")
     ap.add_argument("--workload", choices=("headline", "h2o8", "h2o16-tier", "batch512", "ao2mo",
                                            "copy-calibration"), default="headline",
                     help="headline: 512 x (H2O)2-6 6-31G, full form vs plain FP64 (default); "
                          "h2o8: (H2O)8 cc-pVDZ FP64; h2o16-tier: (H2O)16 cc-pVDZ FP64 vs FP32 tier; "
                          "batch512: 512 small 6-31G clusters through worker processes")
+    ap.add_argument("--pmc-summary", nargs="+", metavar="CSV",
+                    help="print per-kernel counter averages of rocprofv3 --pmc CSV files and exit")
     args = ap.parse_args()
+    if args.pmc_summary:
+        print(pmc_summary(args.pmc_summary))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)

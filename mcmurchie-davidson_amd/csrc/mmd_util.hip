@@ -251,10 +251,10 @@ hipError_t launch_canonical_quartets(int mode, int N, const BFDesc* bfs, const d
 // ---------------------------------------------------------------------------
 // One-electron integrals (cython/onee.pyx:14-192, assembled as
 // mmd/molecule.py:253-276).  One wavefront per basis-function pair i >= j;
-// its lanes take the (primitive pair, nucleus) combinations, so that the
-// nuclear attraction of heavily contracted pairs is spread over the lanes;
-// the overlap, kinetic, dipole and angular-momentum terms are added by the
-// lanes of the first nucleus.  A wavefront reduction sums the lanes.
+// each lane item is a primitive pair with a share of the nuclei, so that the
+// Hermite tables of a primitive pair are built once per item and the nuclear
+// attraction of short contractions is still spread over the lanes.  A
+// wavefront reduction sums the lanes.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) one_electron_kernel(int nbf, const BFDesc* __restrict__ bfs,
                                                            const double* __restrict__ pexp,
@@ -290,10 +290,15 @@ __global__ void __launch_bounds__(256) one_electron_kernel(int nbf, const BFDesc
         at0 = r.x;
         nat_mol = r.y;
     }
-    const int nat = nat_mol > 0 ? nat_mol : 1;
-    const int nwork = a.np * b.np * nat;
-    for (int k = lane; k < nwork; k += 64) {
-        const int at = k % nat, ipp = k / nat;
+    // lane items: a primitive pair and a group g of the molecule's nuclei
+    // (every G-th from g; G = 64 / npp for short contractions, else 1); the
+    // item's Hermite tables serve its nuclei, and the items of group 0 add
+    // the overlap, kinetic and multipole terms
+    const int npp = a.np * b.np;
+    const int G = npp >= 64 ? 1 : 64 / npp;
+    const int nitem = npp * G;
+    for (int it = lane; it < nitem; it += 64) {
+        const int ipp = it % npp, g = it / npp;
         const int ia = ipp / b.np, ib = ipp % b.np;
         const double ea = pexp[a.p0 + ia], eb = pexp[b.p0 + ib];
         const double p = ea + eb;
@@ -312,7 +317,7 @@ __global__ void __launch_bounds__(256) one_electron_kernel(int nbf, const BFDesc
         }
 #define E1(x, ii, jj, tt) \
     (((jj) < 0 || (tt) > (ii) + (jj)) ? 0.0 : E[x][((ii) * (LJ[x] + 1) + (jj)) * NT[x] + (tt)])
-        if (at == 0) {
+        if (g == 0) {
             const double pre = pow(3.141592653589793238462643383279 / p, 1.5) * kab * w;
             double S0[3], S1[3], D1[3], Tk[3], Dp[3];
             for (int x = 0; x < 3; ++x) {
@@ -336,10 +341,11 @@ __global__ void __launch_bounds__(256) one_electron_kernel(int nbf, const BFDesc
             lam[1] += -S0[1] * (S1[2] * D1[0] - S1[0] * D1[2]) * pre;
             lam[2] += -S0[2] * (S1[0] * D1[1] - S1[1] * D1[0]) * pre;
         }
-        // nuclear attraction of nucleus `at` of the pair's molecule
+        // nuclear attraction of every nucleus of the pair's molecule
         // (cython/onee.pyx:175-192)
-        if (at < nat_mol) {
-            const int Lv = l1[0] + l2[0] + l1[1] + l2[1] + l1[2] + l2[2];
+        const int Lv = l1[0] + l2[0] + l1[1] + l2[1] + l1[2] + l2[2];
+        const double vpre = -2.0 * 3.141592653589793238462643383279 / p * kab * w;
+        for (int at = g; at < nat_mol; at += G) {
             const double* nuc = atoms + 4 * (at0 + at);
             const double Z = nuc[0];
             const double X = P[0] - nuc[1], Y = P[1] - nuc[2], W = P[2] - nuc[3];
@@ -355,7 +361,7 @@ __global__ void __launch_bounds__(256) one_electron_kernel(int nbf, const BFDesc
                     for (int vv = 0; vv <= l1[2] + l2[2]; ++vv)
                         val += E1(0, l1[0], l2[0], tt) * E1(1, l1[1], l2[1], uu) * E1(2, l1[2], l2[2], vv)
                              * R[hidx(tt, uu, vv)];
-            v += -Z * val * 2.0 * 3.141592653589793238462643383279 / p * kab * w;
+            v += Z * val * vpre;
         }
 #undef E1
     }

@@ -8,9 +8,10 @@
    local-frame targets.
 2. The perceptron (15 -> 64 -> 64 -> 91, tanh) is trained with libmmd's HIP
    kernels (Adam, mean-squared error on normalised targets).
-3. Held-out set (rng seed 2): a sample is solved from the core, SAD and
-   learned starts (same thread pool, same process); iterations, wall time
-   and energies are reported.
+3. Held-out set (rng seed 2, 2000 geometries): every geometry is solved
+   from the core, SAD and learned starts with the batched device driver;
+   iterations (overall and per cluster size), wall time and energies against
+   the core start are reported.
 
 Writes DIR/learned_guess_631g.npz (the model) and DIR/learned_guess_631g.json
 (the training and evaluation record).  Run it on a machine with the GPU; the
@@ -41,7 +42,10 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--train", type=int, default=20000)
     ap.add_argument("--heldout", type=int, default=2000)
-    ap.add_argument("--evaluate", type=int, default=256, help="held-out geometries solved three ways")
+    ap.add_argument("--evaluate", type=int, default=2000, help="held-out geometries solved three ways")
+    ap.add_argument("--conver", type=float, default=1e-8, help="density convergence of the evaluation runs")
+    ap.add_argument("--model", default=None, help="model to evaluate (default: DIR/learned_guess_631g.npz)")
+    ap.add_argument("--record", default=None, help="evaluation record to write (default: DIR/learned_guess_631g.json)")
     ap.add_argument("--epochs", type=int, default=300)
     ap.add_argument("--procs", type=int, default=8)
     ap.add_argument("--threads", type=int, default=2)
@@ -50,12 +54,13 @@ def main():
                     help="skip data generation and training; evaluate the model already in --out")
     args = ap.parse_args()
     from mmd.batch import run_batch_procs
-    path = os.path.join(args.out, "learned_guess_631g.npz")
-    rec_path = os.path.join(args.out, "learned_guess_631g.json")
+    path = args.model or os.path.join(args.out, "learned_guess_631g.npz")
+    rec_path = args.record or os.path.join(args.out, "learned_guess_631g.json")
     rec = {"train_geometries": args.train, "heldout_geometries": args.heldout, "basis": "6-31g"}
     if args.evaluate_only:
-        if os.path.exists(rec_path):
-            with open(rec_path) as f:
+        old = os.path.join(args.out, "learned_guess_631g.json")
+        if os.path.exists(old):
+            with open(old) as f:
                 rec = json.load(f)
         evaluate(args, path, rec, rec_path)
         return
@@ -91,29 +96,37 @@ def main():
 
 
 def evaluate(args, path, rec, rec_path):
-    """Solve held-out geometries from the core, SAD and learned starts."""
+    """Solve the held-out geometries from the core, SAD and learned starts
+    with the batched device driver (run_batch_device, FP64, groups of 64);
+    iterations, wall time and energies against the core start."""
     from mmd import learned
     learned.DEFAULT_MODEL = path
-    from mmd.batch import run_batch
-    from mmd.guess import sad_density
-    from mmd.molecule import Molecule
+    from mmd.batch import run_batch_device
     heldout = geometries(args.heldout, 2)[:args.evaluate]
-    sad_density(Molecule(heldout[0], "6-31g"))   # atomic densities once, outside the timing
+    run_batch_device(heldout[:128], "6-31g", guess="learned")   # contexts, atomic densities, network: once
     runs = {}
     for guess in ("core", "sad", "learned"):
         t3 = time.perf_counter()
-        runs[guess] = run_batch(heldout, "6-31g", workers=8, guess=guess, conver=1e-8)
+        runs[guess] = run_batch_device(heldout, "6-31g", guess=guess, conver=args.conver)
         runs[guess + "_s"] = time.perf_counter() - t3
-        print("held-out %s done" % guess, flush=True)
+        print("held-out %s done (%.2f s)" % (guess, runs[guess + "_s"]), flush=True)
+    sizes = np.array([len([ln for ln in g.split("\n")[1:] if ln.strip()]) // 3 for g in heldout])
     ev = {}
+    e_core = np.array([r["energy"] for r in runs["core"]])
     for g in ("core", "sad", "learned"):
         its = np.array([r["iterations"] for r in runs[g]])
-        ev[g] = {"mean_iterations": float(its.mean()), "all_converged": bool(all(r["converged"] for r in runs[g])),
-                 "seconds": runs[g + "_s"]}
-    e_core = np.array([r["energy"] for r in runs["core"]])
-    for g in ("sad", "learned"):
-        ev[g]["max_abs_dE_vs_core"] = float(np.max(np.abs(np.array([r["energy"] for r in runs[g]]) - e_core)))
+        e = np.array([r["energy"] for r in runs[g]])
+        ev[g] = {"mean_iterations": float(its.mean()), "median_iterations": float(np.median(its)),
+                 "max_iterations": int(its.max()),
+                 "mean_iterations_by_size": {str(n): float(its[sizes == n].mean()) for n in sorted(set(sizes))},
+                 "all_converged": bool(all(r["converged"] for r in runs[g])), "seconds": runs[g + "_s"]}
+        if g != "core":
+            d = np.abs(e - e_core)
+            ev[g]["max_abs_dE_vs_core"] = float(d.max())
+            ev[g]["within_1e-8_of_core"] = int((d <= 1e-8).sum())
+    ev["learned"]["iterations_saved_vs_sad"] = ev["sad"]["mean_iterations"] - ev["learned"]["mean_iterations"]
     rec["heldout_evaluated"] = len(heldout)
+    rec["evaluation_driver"] = "run_batch_device (FP64, groups of 64), conver %g" % args.conver
     rec["evaluation"] = ev
     with open(rec_path, "w") as f:
         json.dump(rec, f, indent=1)

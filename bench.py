@@ -480,7 +480,8 @@ def bench_headline(args):
     from mmd.molecule import Molecule
     geoms = batch_geometries(args.molecules)
     G = args.group
-    full_kw = dict(group=G, guess="learned", fp32_below=args.fp32_below, streams=args.batch_streams)
+    full_kw = dict(group=G, guess="learned", fp32_below=args.fp32_below, streams=args.batch_streams,
+                   class_streams=args.class_streams)
     for _ in range(max(1, args.warmup)):
         run_batch_device(geoms[:4 * G], "6-31g", **full_kw)
         run_batch_device(geoms[:2 * G], "6-31g", group=G, streams=args.batch_streams)
@@ -505,6 +506,14 @@ def bench_headline(args):
         print("full-form step %d done (%.3f s%s)" % (s, step_s[-1], ", profiled" if prof else ""),
               file=sys.stderr, flush=True)
     el_full = (time.perf_counter() - t0) / args.steps
+    if args.steps_only:   # counter and trace runs: the full-form steps alone, unchecked
+        out = {"metric": METRIC, "value": None, "unit": "SCF/s", "checked": False,
+               "note": "--steps-only: full-form steps without the FP64 and plain checks",
+               "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * el_full, "step_s": step_s,
+               "molecules": len(geoms)}
+        if per:
+            out["roofline"] = headline_roofline(per)
+        return out
     t1 = time.perf_counter()
     ref = run_batch_device(geoms, "6-31g", group=G, streams=args.batch_streams)
     el_64 = time.perf_counter() - t1
@@ -534,7 +543,8 @@ def bench_headline(args):
                                "groups of %d, %d groups in flight, HIP-graph iterations, learned start, FP32 "
                                "tier below %g) vs plain FP64 one at a time" % (len(geoms), G, args.batch_streams, args.fp32_below),
                    "molecules": len(geoms), "seed": 1, "basis": "6-31g", "group": G,
-                   "groups_in_flight": args.batch_streams, "fp32_below": args.fp32_below, "conver": 1e-8,
+                   "groups_in_flight": args.batch_streams, "class_streams": args.class_streams,
+                   "fp32_below": args.fp32_below, "conver": 1e-8,
                    "acc2e": 1e-12,
                    "size_mix": {str(k): ns.count(k) for k in sorted(set(ns))}, "hw_queues": int(HW_QUEUES)},
         "step": "one pass over the %d molecules, geometry strings to converged energies" % len(geoms),
@@ -653,6 +663,9 @@ def main():
     ap.add_argument("--batch-streams", type=int, default=3, help="groups in flight at once (headline)")
     ap.add_argument("--molecules", type=int, default=512,
                     help="headline molecules (512 is the workload; fewer only for counter runs)")
+    ap.add_argument("--class-streams", type=int, default=2,
+                    help="headline: streams per context for the classes of a Fock build (with three groups in "
+                         "flight, two each fill the hardware queues without sharing them)")
     ap.add_argument("--fp32-below", type=float, default=0.0,
                     help="FP32 tier threshold of the headline's full form (0: all FP64; at these sizes the "
                          "tier's second pass over the units costs more than its FP32 arithmetic saves)")
@@ -661,6 +674,8 @@ def main():
                     help="headline: 512 x (H2O)2-6 6-31G, full form vs plain FP64 (default); "
                          "h2o8: (H2O)8 cc-pVDZ FP64; h2o16-tier: (H2O)16 cc-pVDZ FP64 vs FP32 tier; "
                          "batch512: 512 small 6-31G clusters through worker processes")
+    ap.add_argument("--steps-only", action="store_true",
+                    help="headline: run the full-form steps only (for counter and trace runs; no value)")
     ap.add_argument("--pmc-summary", nargs="+", metavar="CSV",
                     help="print per-kernel counter averages of rocprofv3 --pmc CSV files and exit")
     args = ap.parse_args()

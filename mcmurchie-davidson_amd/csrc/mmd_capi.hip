@@ -189,6 +189,7 @@ struct mmd_ctx {
     std::vector<int2> mol_atoms;        // per molecule: first atom, number of atoms
     DevBuf<int2> d_mol_atoms;
     DevBuf<int2> d_plist;
+    DevBuf<int4> d_mol_blocks;          // per molecule: first function, functions, shells [s0, s1)
     int64_t npl = 0;
     int rw = 0;
     // per pair class, per molecule: the class-local range [first, end) of its pairs
@@ -338,17 +339,23 @@ double prim_bound(double p, double K, double AB2, int lab)
 }
 constexpr double PRIM_DROP = 1e-20;
 
-// Shell pairs I >= J in canonical orientation, with primitive-pair data.
-// `qpair` (optional): Schwarz factors; pairs are then sorted inside each
-// pair class by (npp desc, q desc).
-// Shells [mol_s0[m], mol_s0[m+1]) of each molecule (shells never span two).
-void molecule_shell_ranges(mmd_ctx* c)
+// Shells [mol_s0[m], mol_s0[m+1]) of each molecule (shells never span two),
+// and, for a batch, the molecules' blocks {f0, n, s0, s1} on the device.
+int molecule_shell_ranges(mmd_ctx* c)
 {
     const int ns = (int)c->shells.size();
     c->mol_s0.assign(c->nmol + 1, ns);
     for (int I = ns - 1; I >= 0; --I) c->mol_s0[c->shells[I].mol] = I;
     for (int m = c->nmol - 1; m >= 0; --m)
         if (c->mol_s0[m] > c->mol_s0[m + 1]) c->mol_s0[m] = c->mol_s0[m + 1];
+    if (c->nmol > 1) {
+        std::vector<int4> blk(c->nmol);
+        for (int m = 0; m < c->nmol; ++m)
+            blk[m] = make_int4(c->mol_f0[m], c->mol_n[m], c->mol_s0[m], c->mol_s0[m + 1]);
+        HIP_TRY(c->d_mol_blocks.reserve(c->nmol));
+        HIP_TRY(hipMemcpyAsync(c->d_mol_blocks.p, blk.data(), sizeof(int4) * c->nmol, hipMemcpyHostToDevice, c->st));
+    }
+    return MMD_OK;
 }
 
 // Host view of the Schwarz diagonal (pq|pq): element (i, j) = p[row[i] + j].
@@ -364,10 +371,13 @@ struct DiagView {
 // class by (molecule, primitive pairs descending, Schwarz factor
 // descending), with their primitive-pair tables.  diag: the full N x N
 // Schwarz diagonal (pq|pq), or null before it is known (factors 1).
+// Shell pairs I >= J in canonical orientation, with primitive-pair data.
+// `diag` (optional): the Schwarz diagonal for the factors q; pairs are then
+// sorted inside each pair class by (npp desc, q desc).
 int build_pairs(mmd_ctx* c, const DiagView* diag)
 {
     const int N = c->nbf;
-    molecule_shell_ranges(c);
+    if (int rc = molecule_shell_ranges(c)) return rc;
     // largest primitive-pair bound, for the primitive-pair screen
     c->prim_qmax = 0.0;
     for (int m = 0; m < c->nmol; ++m)
@@ -1584,16 +1594,22 @@ int enqueue_fock_sym(mmd_ctx* c, const double* dP, double tol, double* Gout)
 {
     const int N = c->nbf;
     const size_t nn = (size_t)N * N;
-    HIP_TRY(launch_abs_density((int)nn, dP, nullptr, c->d_absD.p, c->st));
-    HIP_TRY(launch_shell_block_max((int)c->shells.size(), c->d_sh_bf0.p, c->d_sh_nbf.p, N, c->d_absD.p, c->d_Dsh.p,
-                                   c->st));
+    const bool blocks = c->nmol > 1;   // a batch: only the molecules' diagonal blocks are live
+    if (blocks) {
+        HIP_TRY(launch_block_prep(c->nmol, c->d_mol_blocks.p, N, dP, c->d_absD.p, c->d_G.p, c->d_sh_bf0.p,
+                                  c->d_sh_nbf.p, (int)c->shells.size(), c->d_Dsh.p, c->st));
+    } else {
+        HIP_TRY(launch_abs_density((int)nn, dP, nullptr, c->d_absD.p, c->st));
+        HIP_TRY(launch_shell_block_max((int)c->shells.size(), c->d_sh_bf0.p, c->d_sh_nbf.p, N, c->d_absD.p,
+                                       c->d_Dsh.p, c->st));
+        HIP_TRY(launch_fill(nn, c->d_G.p, 0.0, c->st));
+    }
     ClassArgs A = base_args(c);
     A.absD = c->d_absD.p;
     A.sqS = c->d_sqS.p;
     A.tol = tol;
     A.D = dP;
     A.G = c->d_G.p;
-    HIP_TRY(launch_fill(nn, c->d_G.p, 0.0, c->st));
     FockScratch W;
     W.rowctr = c->d_rowctr.p;
     W.count = c->d_count.p;
@@ -1605,7 +1621,10 @@ int enqueue_fock_sym(mmd_ctx* c, const double* dP, double tol, double* Gout)
     W.allow_measure = false;
     int rc = run_fock_classes(c, A, tol, W);
     if (rc) return rc;
-    HIP_TRY(launch_symmetrize(N, c->d_G.p, Gout, c->st));
+    if (blocks)
+        HIP_TRY(launch_block_symmetrize(c->nmol, c->d_mol_blocks.p, N, c->d_G.p, Gout, c->st));
+    else
+        HIP_TRY(launch_symmetrize(N, c->d_G.p, Gout, c->st));
     return MMD_OK;
 }
 

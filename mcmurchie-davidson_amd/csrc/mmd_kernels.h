@@ -81,6 +81,13 @@ hipError_t launch_screen_class(int boff, int nb, int koff, int nk, int tri, cons
                                const double* Dsh, int nshell, double tol, int keep_all, int2* surv, int* count,
                                hipStream_t st, double split = 0.0, int2* surv2 = nullptr, int* count2 = nullptr);
 hipError_t launch_abs_density(int n, const double* Dre, const double* Dim, double* absD, hipStream_t st);
+// Block-diagonal batches (blocks: per molecule {first function, functions,
+// first shell, end shell}): |dP|, zero G and shell maxima in the blocks only;
+// the symmetrisation of the blocks.
+hipError_t launch_block_prep(int nmol, const int4* blocks, int N, const double* dP, double* absD, double* G,
+                             const int* sh_bf0, const int* sh_nbf, int nshell, double* Dsh, hipStream_t st);
+hipError_t launch_block_symmetrize(int nmol, const int4* blocks, int N, const double* G, double* out,
+                                   hipStream_t st);
 hipError_t launch_shell_block_max(int nshell, const int* sh_bf0, const int* sh_nbf, int N, const double* absD,
                                   double* Dsh, hipStream_t st);
 hipError_t launch_count_prims(const int2* surv, const int* count, const ShellPair* pairs, unsigned long long* out,

@@ -921,6 +921,60 @@ hipError_t launch_symmetrize(int N, const double* G, double* out, hipStream_t st
     return hipGetLastError();
 }
 
+// Block-diagonal batch: the per-build preparation restricted to each
+// molecule's diagonal block (one workgroup per molecule): |dP| and a zero G
+// in the block, then the shell-block maxima of |dP| over the molecule's
+// shell pairs.  Entries outside the blocks are never read by the builds.
+__global__ void __launch_bounds__(256) block_prep_kernel(const int4* __restrict__ blocks, int N,
+                                                         const double* __restrict__ dP, double* __restrict__ absD,
+                                                         double* __restrict__ G, const int* __restrict__ bf0,
+                                                         const int* __restrict__ nbf, int nshell,
+                                                         double* __restrict__ Dsh)
+{
+    const int4 b = blocks[blockIdx.x];
+    const int f0 = b.x, n = b.y, s0 = b.z, ns = b.w - b.z;
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+        const size_t o = (size_t)(f0 + e / n) * N + f0 + e % n;
+        absD[o] = fabs(dP[o]);
+        G[o] = 0.0;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < ns * ns; e += blockDim.x) {
+        const int s1 = s0 + e / ns, s2 = s0 + e % ns;
+        double m = 0.0;
+        for (int i = 0; i < nbf[s1]; ++i)
+            for (int j = 0; j < nbf[s2]; ++j) m = fmax(m, absD[(size_t)(bf0[s1] + i) * N + bf0[s2] + j]);
+        Dsh[(size_t)s1 * nshell + s2] = m;
+    }
+}
+
+hipError_t launch_block_prep(int nmol, const int4* blocks, int N, const double* dP, double* absD, double* G,
+                             const int* sh_bf0, const int* sh_nbf, int nshell, double* Dsh, hipStream_t st)
+{
+    if (nmol <= 0) return hipSuccess;
+    hipLaunchKernelGGL(block_prep_kernel, dim3(nmol), dim3(256), 0, st, blocks, N, dP, absD, G, sh_bf0, sh_nbf,
+                       nshell, Dsh);
+    return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256) block_symmetrize_kernel(const int4* __restrict__ blocks, int N,
+                                                               const double* __restrict__ G, double* __restrict__ out)
+{
+    const int4 b = blocks[blockIdx.x];
+    const int f0 = b.x, n = b.y;
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+        const int i = f0 + e / n, j = f0 + e % n;
+        out[(size_t)i * N + j] = 0.5 * (G[(size_t)i * N + j] + G[(size_t)j * N + i]);
+    }
+}
+
+hipError_t launch_block_symmetrize(int nmol, const int4* blocks, int N, const double* G, double* out, hipStream_t st)
+{
+    if (nmol <= 0) return hipSuccess;
+    hipLaunchKernelGGL(block_symmetrize_kernel, dim3(nmol), dim3(256), 0, st, blocks, N, G, out);
+    return hipGetLastError();
+}
+
 __global__ void sqrt_kernel(size_t n, const double* __restrict__ x, double* __restrict__ y)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

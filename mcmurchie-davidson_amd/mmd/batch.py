@@ -343,14 +343,16 @@ def _device_contexts(device, count):
 
 def run_batch_device(geometries, basis="6-31g", group=64, guess="core", fp32_below=0.0, conver=1e-8,
                      acc2e=1e-12, device=0, use_graph=True, keep_density=False, guess_model=None, ctx=None,
-                     streams=2):
+                     streams=2, class_streams=None):
     """Direct RHF energies of all geometries, `group` molecules at a time in
     one block-diagonal basis on the GPU (mmd_batch_rhf).  guess: 'core' (the
     reference's), 'sad' or 'learned' (initial densities formed for the whole
     group on the host side, the network on the GPU).  fp32_below: the FP32
     tier of the Fock builds.  streams: groups in flight at once, each on its
     own context (stream), so that one group's host-side set-up overlaps
-    another's device work; with ctx given, that context alone.  Returns one
+    another's device work; with ctx given, that context alone.
+    class_streams: streams over which each context spreads the classes of
+    a Fock build (default: the context's setting).  Returns one
     dict per geometry: energy, iterations, converged, nbasis (and the density
     P with keep_density)."""
     if group < 1:
@@ -362,6 +364,9 @@ def run_batch_device(geometries, basis="6-31g", group=64, guess="core", fp32_bel
         guess_model = guess_model or default_model()
     groups = [list(range(g0, min(len(geometries), g0 + group))) for g0 in range(0, len(geometries), group)]
     ctxs = [ctx] if ctx is not None else _device_contexts(device, max(1, min(int(streams), len(groups))))
+    if class_streams is not None:
+        for c in ctxs:
+            c.set_streams(int(class_streams))
     out = [None] * len(geometries)
     args = (basis, guess, fp32_below, conver, acc2e, use_graph, keep_density, guess_model, out)
     if len(ctxs) == 1:

@@ -73,6 +73,8 @@ def lib():
                                         ctypes.c_void_p, _i64, _i64]
         L.orc_fock_elements.restype = None
         L.orc_fock_elements.argtypes = bfs + [_i64, _pd, _pd, _d, _i64, _pi, _pd]
+        L.orc_rhf_gradient.restype = None
+        L.orc_rhf_gradient.argtypes = bfs + [_i64, _i64, _pd, _pd, _pi, _pd, _pd, _pd]
         L.orc_one_electron.restype = None
         L.orc_one_electron.argtypes = bfs + [_i64, _i64, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd]
         _lib = L
@@ -177,18 +179,20 @@ class OracleMolecule:
         self.nelec = sum(Z for Z, _ in self.atoms) - self.charge
         self.nocc = self.nelec // 2
         data = read_gbs(basis_path)
-        origin, shell, nps, exps, coefs = [], [], [], [], []
-        for Z, xyz in self.atoms:
+        origin, shell, nps, exps, coefs, bf_atom = [], [], [], [], [], []
+        for ia, (Z, xyz) in enumerate(self.atoms):
             for mom, prims in data[Z]:
                 e = [p[0] for p in prims]
                 c = [p[1] for p in prims]
                 for lmn in SHELLS[mom]:
+                    bf_atom.append(ia)
                     origin.append(xyz)
                     shell.append(lmn)
                     nps.append(len(e))
                     exps.extend(e)
                     coefs.extend(c)
         self.nbasis = len(shell)
+        self.bf_atom = np.array(bf_atom, dtype=np.int64)
         self.origin = np.ascontiguousarray(np.array(origin, dtype=np.float64).reshape(-1, 3))
         self.shell = np.ascontiguousarray(np.array(shell, dtype=np.int64).reshape(-1, 3))
         self.np = np.array(nps, dtype=np.int64)
@@ -217,6 +221,17 @@ class OracleMolecule:
 
     def _bfs(self):
         return (self.origin, self.shell, self.np, self.poff, self.exps, self.coefs, self.norm)
+
+    def rhf_gradient(self, P, F):
+        """dE/dX, shape (natom, 3), of the converged RHF energy for density P
+        and Fock matrix F (mmd/forces.py:8-97, cython/grad.pyx); the
+        reference's forces are its negative."""
+        N = self.nbasis
+        P = np.ascontiguousarray(np.real(P), dtype=np.float64)
+        W = np.ascontiguousarray(P @ np.real(F) @ P)
+        g = np.zeros((len(self.atoms), 3))
+        lib().orc_rhf_gradient(*self._bfs(), N, len(self.atoms), self.Z, self.xyz, self.bf_atom, P, W, g)
+        return g
 
     def one_electron(self):
         N = self.nbasis

@@ -562,3 +562,235 @@ void orc_one_electron(ORC_BFS_ARGS, int64_t N, int64_t natom, const double *Z,
             }
         }
 }
+
+/* ------------------------------------------------------------------ */
+/* Nuclear gradient of the RHF energy (mmd/forces.py:8-97 with the      */
+/* derivative integrals of cython/grad.pyx:11-350).                     */
+/* ------------------------------------------------------------------ */
+/* First derivative of a 1-D Hermite coefficient with respect to the    */
+/* centre of the first (q = 1) or second (r = 1) Gaussian               */
+/* (cython/grad.pyx:97-103).                                            */
+static double orc_Ex(int i, int j, int t, double Q, double a, double b, int q, int r)
+{
+    if (q == 1) return 2.0 * a * orc_E(i + 1, j, t, Q, a, b, 0, 0.0) - i * orc_E(i - 1, j, t, Q, a, b, 0, 0.0);
+    if (r == 1) return 2.0 * b * orc_E(i, j + 1, t, Q, a, b, 0, 0.0) - j * orc_E(i, j - 1, t, Q, a, b, 0, 0.0);
+    return orc_E(i, j, t, Q, a, b, 0, 0.0);
+}
+
+/* 1-D coefficient along axis x of a primitive pair: plain, or the      */
+/* derivative for the centre `ctr` (1 = first, 2 = second) when x is    */
+/* the differentiated axis `dx`.                                        */
+static double coef1(int x, int dx, int ctr, const int64_t *l1, const int64_t *l2, int dj, int t,
+                    const double *A, const double *B, double a, double b)
+{
+    int i = (int)l1[x], j = (int)l2[x] + dj;
+    if (j < 0) return 0.0;
+    if (x == dx && ctr == 1) return orc_Ex(i, j, t, A[x] - B[x], a, b, 1, 0);
+    if (x == dx && ctr == 2) return orc_Ex(i, j, t, A[x] - B[x], a, b, 0, 1);
+    return orc_E(i, j, t, A[x] - B[x], a, b, 0, 0.0);
+}
+
+/* overlapX (cython/grad.pyx:352-398) */
+static double prim_overlap_x(double a, const int64_t *l1, const double *A, double b, const int64_t *l2,
+                             const double *B, int dx, int ctr)
+{
+    double s = 1.0;
+    for (int x = 0; x < 3; ++x) s *= coef1(x, dx, ctr, l1, l2, 0, 0, A, B, a, b);
+    return s * pow(PI / (a + b), 1.5);
+}
+
+/* kineticX (cython/grad.pyx:400-507) */
+static double prim_kinetic_x(double a, const int64_t *l1, const double *A, double b, const int64_t *l2,
+                             const double *B, int dx, int ctr)
+{
+    double Tk[3], S0[3];
+    for (int x = 0; x < 3; ++x) {
+        double j = (double)l2[x];
+        Tk[x] = (2.0 * j + 1.0) * b * coef1(x, dx, ctr, l1, l2, 0, 0, A, B, a, b)
+              - 2.0 * b * b * coef1(x, dx, ctr, l1, l2, 2, 0, A, B, a, b)
+              - 0.5 * j * (j - 1.0) * coef1(x, dx, ctr, l1, l2, -2, 0, A, B, a, b);
+        S0[x] = coef1(x, dx, ctr, l1, l2, 0, 0, A, B, a, b);
+    }
+    return (Tk[0] * S0[1] * S0[2] + Tk[1] * S0[0] * S0[2] + Tk[2] * S0[0] * S0[1]) * pow(PI / (a + b), 1.5);
+}
+
+/* nuclear_attractionXa (operator derivative, cython/grad.pyx:509-551) and
+ * nuclear_attractionXb (basis-function derivative, 553-): both without
+ * the nuclear charge. */
+static double prim_nuclear_xa(double a, const int64_t *l1, const double *A, double b, const int64_t *l2,
+                              const double *B, const double *C, int dx)
+{
+    double p = a + b, P[3];
+    for (int x = 0; x < 3; ++x) P[x] = (a * A[x] + b * B[x]) / p;
+    double RPC = sqrt((P[0] - C[0]) * (P[0] - C[0]) + (P[1] - C[1]) * (P[1] - C[1]) + (P[2] - C[2]) * (P[2] - C[2]));
+    double val = 0.0;
+    for (int t = 0; t <= l1[0] + l2[0]; ++t)
+        for (int u = 0; u <= l1[1] + l2[1]; ++u)
+            for (int v = 0; v <= l1[2] + l2[2]; ++v)
+                val -= orc_E((int)l1[0], (int)l2[0], t, A[0] - B[0], a, b, 0, 0.0)
+                     * orc_E((int)l1[1], (int)l2[1], u, A[1] - B[1], a, b, 0, 0.0)
+                     * orc_E((int)l1[2], (int)l2[2], v, A[2] - B[2], a, b, 0, 0.0)
+                     * orc_R(t + (dx == 0), u + (dx == 1), v + (dx == 2), 0, p, P[0] - C[0], P[1] - C[1],
+                             P[2] - C[2], RPC);
+    return val * 2.0 * PI / p;
+}
+
+static double prim_nuclear_xb(double a, const int64_t *l1, const double *A, double b, const int64_t *l2,
+                              const double *B, const double *C, int dx, int ctr)
+{
+    double p = a + b, P[3];
+    for (int x = 0; x < 3; ++x) P[x] = (a * A[x] + b * B[x]) / p;
+    double RPC = sqrt((P[0] - C[0]) * (P[0] - C[0]) + (P[1] - C[1]) * (P[1] - C[1]) + (P[2] - C[2]) * (P[2] - C[2]));
+    double val = 0.0;
+    for (int t = 0; t <= l1[0] + l2[0] + (dx == 0); ++t)
+        for (int u = 0; u <= l1[1] + l2[1] + (dx == 1); ++u)
+            for (int v = 0; v <= l1[2] + l2[2] + (dx == 2); ++v)
+                val += coef1(0, dx, ctr, l1, l2, 0, t, A, B, a, b) * coef1(1, dx, ctr, l1, l2, 0, u, A, B, a, b)
+                     * coef1(2, dx, ctr, l1, l2, 0, v, A, B, a, b)
+                     * orc_R(t, u, v, 0, p, P[0] - C[0], P[1] - C[1], P[2] - C[2], RPC);
+    return val * 2.0 * PI / p;
+}
+
+/* electron_repulsionX (cython/grad.pyx:107-350): the derivative of the
+ * primitive [ab|cd] with respect to centre `ctr` (0..3 = a..d) along dx. */
+static double prim_eri_x(double ea, const int64_t *l1, const double *A, double eb, const int64_t *l2,
+                         const double *B, double ec, const int64_t *l3, const double *C, double ed,
+                         const int64_t *l4, const double *D, int dx, int ctr)
+{
+    double p = ea + eb, q = ec + ed, alpha = p * q / (p + q);
+    double P[3], Q[3];
+    for (int x = 0; x < 3; ++x) {
+        P[x] = (ea * A[x] + eb * B[x]) / p;
+        Q[x] = (ec * C[x] + ed * D[x]) / q;
+    }
+    double RPQ = sqrt((P[0] - Q[0]) * (P[0] - Q[0]) + (P[1] - Q[1]) * (P[1] - Q[1]) + (P[2] - Q[2]) * (P[2] - Q[2]));
+    int cb = ctr == 0 ? 1 : (ctr == 1 ? 2 : 0);   /* bra centre derivative */
+    int ck = ctr == 2 ? 1 : (ctr == 3 ? 2 : 0);   /* ket centre derivative */
+    int nb[3], nk[3];
+    for (int x = 0; x < 3; ++x) {
+        nb[x] = (int)(l1[x] + l2[x]) + (cb && x == dx);
+        nk[x] = (int)(l3[x] + l4[x]) + (ck && x == dx);
+    }
+    double val = 0.0;
+    for (int t = 0; t <= nb[0]; ++t)
+        for (int u = 0; u <= nb[1]; ++u)
+            for (int v = 0; v <= nb[2]; ++v) {
+                double eb3 = coef1(0, dx, cb, l1, l2, 0, t, A, B, ea, eb) * coef1(1, dx, cb, l1, l2, 0, u, A, B, ea, eb)
+                           * coef1(2, dx, cb, l1, l2, 0, v, A, B, ea, eb);
+                if (eb3 == 0.0) continue;
+                for (int tau = 0; tau <= nk[0]; ++tau)
+                    for (int nu = 0; nu <= nk[1]; ++nu)
+                        for (int phi = 0; phi <= nk[2]; ++phi)
+                            val += eb3 * coef1(0, dx, ck, l3, l4, 0, tau, C, D, ec, ed)
+                                 * coef1(1, dx, ck, l3, l4, 0, nu, C, D, ec, ed)
+                                 * coef1(2, dx, ck, l3, l4, 0, phi, C, D, ec, ed)
+                                 * (((tau + nu + phi) & 1) ? -1.0 : 1.0)
+                                 * orc_R(t + tau, u + nu, v + phi, 0, alpha, P[0] - Q[0], P[1] - Q[1], P[2] - Q[2],
+                                         RPQ);
+            }
+    return val * 2.0 * pow(PI, 2.5) / (p * q * sqrt(p + q));
+}
+
+/* dE/dX of the converged RHF energy, natom x 3 (the reference stores
+ * -dE/dX as the force, mmd/forces.py:93-97):
+ *   sum_pq P_qp (2 Hx + Gx)_pq - 2 sum_pq Sx_pq W_qp + dV_NN/dX,
+ * Gx = 2 Jx - Kx from the derivative ERIs, W = P F P (forces.py:81-91).
+ * The derivative tensor is never stored: each canonical quartet's
+ * derivatives (the four centres' terms of forces.py:68-71) are weighted by
+ * the density products of the positions it would fill in dTwoEx
+ * (forces.py:73-80), which is the same sum as the einsums of 83-84.
+ * bf_atom: the atom of each basis function (atom.mask, molecule.py:74-80). */
+void orc_rhf_gradient(ORC_BFS_ARGS, int64_t N, int64_t natom, const double *Z, const double *axyz,
+                      const int64_t *bf_atom, const double *P, const double *W, double *grad)
+{
+    ORC_BFS_INIT;
+    memset(grad, 0, sizeof(double) * 3 * natom);
+#define Pm(a, b) P[(a) * N + (b)]
+    /* one-electron terms, per pair i >= j (weight 2 off the diagonal) */
+    for (int64_t i = 0; i < N; ++i)
+        for (int64_t j = 0; j <= i; ++j) {
+            double f = (i == j) ? 1.0 : 2.0;
+            int64_t oi = poff[i], oj = poff[j];
+            const int64_t *l1 = shell + 3 * i, *l2 = shell + 3 * j;
+            const double *A = origin + 3 * i, *Bc = origin + 3 * j;
+            for (int dx = 0; dx < 3; ++dx) {
+                double sA = 0, sB = 0, hA = 0, hB = 0;
+                double *hC = (double *)calloc((size_t)natom, sizeof(double));
+                for (int64_t ia = 0; ia < np[i]; ++ia)
+                    for (int64_t ib = 0; ib < np[j]; ++ib) {
+                        double a = exps[oi + ia], b = exps[oj + ib];
+                        double w = norm[oi + ia] * norm[oj + ib] * coefs[oi + ia] * coefs[oj + ib];
+                        sA += w * prim_overlap_x(a, l1, A, b, l2, Bc, dx, 1);
+                        sB += w * prim_overlap_x(a, l1, A, b, l2, Bc, dx, 2);
+                        hA += w * prim_kinetic_x(a, l1, A, b, l2, Bc, dx, 1);
+                        hB += w * prim_kinetic_x(a, l1, A, b, l2, Bc, dx, 2);
+                        for (int64_t c = 0; c < natom; ++c) {
+                            const double *C = axyz + 3 * c;
+                            hA -= w * Z[c] * prim_nuclear_xb(a, l1, A, b, l2, Bc, C, dx, 1);
+                            hB -= w * Z[c] * prim_nuclear_xb(a, l1, A, b, l2, Bc, C, dx, 2);
+                            hC[c] -= w * Z[c] * prim_nuclear_xa(a, l1, A, b, l2, Bc, C, dx);
+                        }
+                    }
+                grad[3 * bf_atom[i] + dx] += f * (2.0 * Pm(i, j) * hA - 2.0 * W[i * N + j] * sA);
+                grad[3 * bf_atom[j] + dx] += f * (2.0 * Pm(i, j) * hB - 2.0 * W[i * N + j] * sB);
+                for (int64_t c = 0; c < natom; ++c) grad[3 * c + dx] += f * 2.0 * Pm(i, j) * hC[c];
+                free(hC);
+            }
+        }
+    /* two-electron terms over canonical quartets */
+    for (int64_t i = 0; i < N; ++i)
+        for (int64_t j = 0; j <= i; ++j)
+            for (int64_t k = 0; k < N; ++k)
+                for (int64_t l = 0; l <= k; ++l) {
+                    int64_t ij = i * (i + 1) / 2 + j, kl = k * (k + 1) / 2 + l;
+                    if (ij < kl) continue;
+                    /* the distinct positions of the quartet's orbit */
+                    int64_t perm[8][4] = {{i, j, k, l}, {k, l, i, j}, {j, i, l, k}, {l, k, j, i},
+                                          {j, i, k, l}, {l, k, i, j}, {i, j, l, k}, {k, l, j, i}};
+                    double wgt = 0.0;
+                    for (int s = 0; s < 8; ++s) {
+                        int dup = 0;
+                        for (int r = 0; r < s && !dup; ++r)
+                            dup = perm[r][0] == perm[s][0] && perm[r][1] == perm[s][1] && perm[r][2] == perm[s][2]
+                               && perm[r][3] == perm[s][3];
+                        if (dup) continue;
+                        int64_t a = perm[s][0], b = perm[s][1], c = perm[s][2], d = perm[s][3];
+                        wgt += 2.0 * Pm(b, a) * Pm(d, c) - Pm(c, a) * Pm(b, d);
+                    }
+                    if (wgt == 0.0) continue;
+                    const int64_t idx[4] = {i, j, k, l};
+                    for (int ctr = 0; ctr < 4; ++ctr)
+                        for (int dx = 0; dx < 3; ++dx) {
+                            double d = 0.0;
+                            int64_t o[4];
+                            for (int s = 0; s < 4; ++s) o[s] = poff[idx[s]];
+                            for (int64_t ja = 0; ja < np[i]; ++ja)
+                                for (int64_t jb = 0; jb < np[j]; ++jb)
+                                    for (int64_t jc = 0; jc < np[k]; ++jc)
+                                        for (int64_t jd = 0; jd < np[l]; ++jd)
+                                            d += norm[o[0] + ja] * norm[o[1] + jb] * norm[o[2] + jc] * norm[o[3] + jd]
+                                               * coefs[o[0] + ja] * coefs[o[1] + jb] * coefs[o[2] + jc]
+                                               * coefs[o[3] + jd]
+                                               * prim_eri_x(exps[o[0] + ja], shell + 3 * i, origin + 3 * i,
+                                                            exps[o[1] + jb], shell + 3 * j, origin + 3 * j,
+                                                            exps[o[2] + jc], shell + 3 * k, origin + 3 * k,
+                                                            exps[o[3] + jd], shell + 3 * l, origin + 3 * l, dx, ctr);
+                            grad[3 * bf_atom[idx[ctr]] + dx] += wgt * d;
+                        }
+                }
+#undef Pm
+    (void)B;
+    /* nuclear repulsion (forces.py:52-60) */
+    for (int64_t a = 0; a < natom; ++a)
+        for (int64_t c = 0; c < natom; ++c) {
+            if (a == c) continue;
+            double dx3[3], r2 = 0.0;
+            for (int x = 0; x < 3; ++x) {
+                dx3[x] = axyz[3 * a + x] - axyz[3 * c + x];
+                r2 += dx3[x] * dx3[x];
+            }
+            double r = sqrt(r2);
+            if (r < 1e-12) continue;
+            for (int x = 0; x < 3; ++x) grad[3 * a + x] += -dx3[x] * Z[a] * Z[c] / (r2 * r);
+        }
+}

@@ -424,3 +424,49 @@ def test_cluster_generator():
     d = np.linalg.norm(O[:, None] - O[None], axis=-1) + np.eye(8) * 10
     assert d.min() >= 2.7 and O.min() >= 0 and O.max() <= 7.0 + 1.2
     assert water_cluster(8, 7.0, 0) == g
+
+
+# --------------------------------------------------------------------------
+# Nuclear gradient (mmd/forces.py:8-97, cython/grad.pyx)
+# --------------------------------------------------------------------------
+WATER_BENT = """0 1
+O 0.05 -0.07 0.01
+H 0.86 0.62 -0.03
+H -0.88 0.58 0.04
+"""
+
+
+def test_gradient_h2_reference(oracle):
+    """H2 / STO-3G forces of tests/test001.py:15-18, with that test's
+    assert_allclose(atol=1e-12) (and numpy's default rtol)."""
+    from numpy.testing import assert_allclose
+    om = oracle.OracleMolecule(H2, _gbs("sto-3g"))
+    r = oracle.rhf(om)
+    assert abs(r["energy"].real - (-1.11675930740)) < 1e-10
+    forces = -om.rhf_gradient(r["P"], r["F"])
+    ref = np.array([[0.000000000, -0.000000000, -0.027679601], [-0.000000000, 0.000000000, 0.027679601]])
+    assert_allclose(forces, ref, atol=1e-12)
+
+
+@pytest.mark.parametrize("basis", ["sto-3g", "6-31g"])
+def test_gradient_finite_differences(oracle, basis):
+    """The analytic gradient of a bent, off-axis water against central
+    differences of the converged energy (h = 1e-4 bohr; the O(h^2) error is
+    about 1e-9), and its translational and rotational invariance."""
+    om = oracle.OracleMolecule(WATER_BENT, _gbs(basis))
+    r = oracle.rhf(om, conver=1e-11)
+    g = om.rhf_gradient(r["P"], r["F"])
+    assert np.max(np.abs(g.sum(0))) < 1e-12
+    assert np.max(np.abs(np.cross(om.xyz, g).sum(0))) < 1e-12
+    h = 1e-4
+    for a, x in ((0, 0), (1, 1), (2, 2), (0, 1)):
+        e = []
+        for sgn in (1, -1):
+            lines = WATER_BENT.strip().split("\n")
+            parts = lines[1 + a].split()
+            vals = [float(v) for v in parts[1:]]
+            vals[x] += sgn * h * 0.52917721092
+            lines[1 + a] = parts[0] + " " + " ".join("%.12f" % v for v in vals)
+            e.append(oracle.rhf(oracle.OracleMolecule("\n".join(lines) + "\n", _gbs(basis)), conver=1e-11)["energy"].real)
+        fd = (e[0] - e[1]) / (2 * h)
+        assert abs(g[a, x] - fd) < 1e-8, (a, x, g[a, x], fd)

@@ -165,6 +165,22 @@ int mmd_jk_tensor(mmd_ctx *ctx, const double *T, int t_on_device, const double *
 int mmd_boys(mmd_ctx *ctx, int n, const double *m, const double *T, int table, double *out);
 
 /*
+ * Electronic part of the nuclear gradient of a converged RHF energy,
+ * replacing the derivative-integral loops of Forces.forces
+ * (mmd/forces.py:8-97; integrals cython/grad.pyx:11-350):
+ *   grad[3*A + x] = sum_pq P_pq 2 dH_pq/dX_Ax + sum_pqrs Gamma_pqrs d(pq|rs)/dX_Ax
+ *                   - 2 sum_pq W_pq dS_pq/dX_Ax
+ * for the context's basis and nuclei (mmd_set_atoms); P the half density
+ * C_occ C_occ^T and W = P F P (N x N, host), bf_atom the atom of each basis
+ * function (the reference's atom.mask, mmd/molecule.py:74-80).  The nuclear
+ * repulsion term (forces.py:52-60) is the caller's.  grad: natom x 3, host.
+ * One molecule per context; up to 4096 atoms.  mmd_last_gradient_ms: device
+ * time of the last call (HIP events around its kernels).
+ */
+int mmd_rhf_gradient(mmd_ctx *ctx, const double *P, const double *W, const int32_t *bf_atom, double *grad);
+int mmd_last_gradient_ms(mmd_ctx *ctx, double *ms);
+
+/*
  * Per-class statistics of the symmetrised Fock build.  With profiling on,
  * every mmd_fock_delta(raw = 0) records, per shell-quartet class
  * (la lb | lc ld): surviving shell quartets, surviving primitive shell

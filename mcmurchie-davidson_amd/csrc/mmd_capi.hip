@@ -178,6 +178,9 @@ struct mmd_ctx {
     // atoms
     int natom = 0;
     DevBuf<double> d_atoms;
+    DevBuf<double> g_PW;                // gradient: P, W and the natom x 3 result
+    DevBuf<int> g_atom;                 // gradient: atom of each basis function
+    double last_grad_ms = 0.0;
 
     // Molecules of a block-diagonal basis (mmd_set_molecules; one molecule
     // unless set): molecule m owns functions [mol_f0[m], mol_f0[m] + mol_n[m])
@@ -1097,6 +1100,47 @@ int mmd_set_molecules(mmd_ctx* ctx, int nmol, const int32_t* nbf_per_mol, const 
     ctx->pairs_ok = false;
     ctx->schwarz_ok = false;
     ctx->units_ok = false;
+    return MMD_OK;
+}
+
+int mmd_rhf_gradient(mmd_ctx* ctx, const double* P, const double* W, const int32_t* bf_atom, double* grad)
+{
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    const int N = ctx->nbf, natom = ctx->natom;
+    if (N == 0) return fail(MMD_ERR_STATE, "no basis set");
+    if (natom < 1) return fail(MMD_ERR_STATE, "no nuclei (mmd_set_atoms)");
+    if (ctx->nmol > 1) return fail(MMD_ERR_UNSUPPORTED, "the gradient takes one molecule, not a batch");
+    if (natom > 4096) return fail(MMD_ERR_UNSUPPORTED, "the gradient takes up to 4096 atoms");
+    if (!P || !W || !bf_atom || !grad) return fail(MMD_ERR_ARG, "null pointer");
+    for (int i = 0; i < N; ++i)
+        if (bf_atom[i] < 0 || bf_atom[i] >= natom) return fail(MMD_ERR_ARG, "basis function %d: atom out of range", i);
+    const size_t nn = (size_t)N * N;
+    HIP_TRY(ctx->g_PW.reserve(2 * nn + 3 * (size_t)natom));
+    HIP_TRY(ctx->g_atom.reserve(N));
+    double* dP = ctx->g_PW.p;
+    double* dW = dP + nn;
+    double* dg = dW + nn;
+    HIP_TRY(hipMemcpyAsync(dP, P, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->st));
+    HIP_TRY(hipMemcpyAsync(dW, W, sizeof(double) * nn, hipMemcpyHostToDevice, ctx->st));
+    HIP_TRY(hipMemcpyAsync(ctx->g_atom.p, bf_atom, sizeof(int32_t) * N, hipMemcpyHostToDevice, ctx->st));
+    HIP_TRY(launch_fill(3 * (size_t)natom, dg, 0.0, ctx->st));
+    HIP_TRY(hipEventRecord(ctx->ev0, ctx->st));
+    HIP_TRY(launch_rhf_gradient(N, ctx->d_bf.p, ctx->d_bf_exp.p, ctx->d_bf_w.p, natom, ctx->d_atoms.p, ctx->d_boys.p,
+                                ctx->g_atom.p, dP, dW, dg, ctx->st));
+    HIP_TRY(hipEventRecord(ctx->ev1, ctx->st));
+    HIP_TRY(hipMemcpyAsync(grad, dg, sizeof(double) * 3 * natom, hipMemcpyDeviceToHost, ctx->st));
+    HIP_TRY(hipStreamSynchronize(ctx->st));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_grad_ms = ms;
+    return MMD_OK;
+}
+
+int mmd_last_gradient_ms(mmd_ctx* ctx, double* ms)
+{
+    if (!ctx || !ms) return fail(MMD_ERR_ARG, "null pointer");
+    *ms = ctx->last_grad_ms;
     return MMD_OK;
 }
 

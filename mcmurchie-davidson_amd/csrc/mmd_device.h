@@ -95,7 +95,7 @@ constexpr double BOYS_INV_DT = 10.0;
 constexpr int BOYS_NT = 361;          // T in [0, 36.05]
 constexpr double BOYS_TMAX = 36.0;
 constexpr int BOYS_TAYLOR = 7;        // terms k = 0..6
-constexpr int BOYS_MMAX = MAX_LTOT + BOYS_TAYLOR - 1;   // 14
+constexpr int BOYS_MMAX = MAX_LTOT + BOYS_TAYLOR;       // 15: derivative integrals reach L + 1
 constexpr int BOYS_ROWS = BOYS_MMAX + 1;
 
 __device__ __forceinline__ double dev_rsqrt(double x) { return rsqrt(x); }

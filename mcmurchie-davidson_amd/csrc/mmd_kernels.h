@@ -81,6 +81,11 @@ hipError_t launch_screen_class(int boff, int nb, int koff, int nk, int tri, cons
                                const double* Dsh, int nshell, double tol, int keep_all, int2* surv, int* count,
                                hipStream_t st, double split = 0.0, int2* surv2 = nullptr, int* count2 = nullptr);
 hipError_t launch_abs_density(int n, const double* Dre, const double* Dim, double* absD, hipStream_t st);
+// Electronic part of the RHF nuclear gradient (natom x 3, added to grad):
+// P, W = P F P (N x N), bf_atom (N); atoms natom x 4 (Z, x, y, z).
+hipError_t launch_rhf_gradient(int nbf, const BFDesc* bfs, const double* pexp, const double* pw, int natom,
+                               const double* atoms, const double* boys, const int* bf_atom, const double* P,
+                               const double* W, double* grad, hipStream_t st);
 // Block-diagonal batches (blocks: per molecule {first function, functions,
 // first shell, end shell}): |dP|, zero G and shell maxima in the blocks only;
 // the symmetrisation of the blocks.

@@ -84,6 +84,10 @@ lib.mmd_set_profiling.restype = _c_int
 lib.mmd_set_profiling.argtypes = [_ctx_p, _c_int]
 lib.mmd_fock_stats.restype = _c_int
 lib.mmd_fock_stats.argtypes = [_ctx_p, _vp, _vp, _vp, _vp]
+lib.mmd_rhf_gradient.restype = _c_int
+lib.mmd_rhf_gradient.argtypes = [_ctx_p, _vp, _vp, _vp, _vp]
+lib.mmd_last_gradient_ms.restype = _c_int
+lib.mmd_last_gradient_ms.argtypes = [_ctx_p, _vp]
 lib.mmd_fock_profile_sums.restype = _c_int
 lib.mmd_fock_profile_sums.argtypes = [_ctx_p, _vp, _vp, _vp, _vp, _vp]
 lib.mmd_last_fock_ms.restype = _c_int
@@ -125,7 +129,8 @@ EXPORTED = [
     "mmd_last_error", "mmd_version", "mmd_device_count", "mmd_ctx_create", "mmd_ctx_destroy",
     "mmd_set_basis", "mmd_set_atoms", "mmd_one_electron", "mmd_schwarz", "mmd_set_schwarz",
     "mmd_eri_tensor", "mmd_eri_list", "mmd_fock_delta", "mmd_fock_delta_noalloc", "mmd_jk_tensor", "mmd_boys",
-    "mmd_last_fock_ms", "mmd_set_profiling", "mmd_fock_stats", "mmd_fock_profile_sums", "mmd_set_precision", "mmd_fock_tiers",
+    "mmd_last_fock_ms", "mmd_set_profiling", "mmd_fock_stats", "mmd_fock_profile_sums", "mmd_rhf_gradient",
+    "mmd_last_gradient_ms", "mmd_set_precision", "mmd_fock_tiers",
     "mmd_set_streams", "mmd_set_lds_limit", "mmd_overlap_powers", "mmd_dgemm", "mmd_set_molecules",
     "mmd_batch_rhf", "mmd_scf_begin", "mmd_scf_step", "mmd_scf_density", "mmd_scf_get", "mmd_ao2mo",
     "mmd_mlp_create", "mmd_mlp_destroy", "mmd_mlp_nparam", "mmd_mlp_set_params", "mmd_mlp_get_params",
@@ -206,6 +211,7 @@ class Context:
         Z = _f64(Z)
         xyz = _f64(xyz).reshape(-1, 3)
         check(lib.mmd_set_atoms(self._h, len(Z), _ptr(Z), _ptr(xyz)))
+        self.natom = len(Z)
 
     # -- integrals -------------------------------------------------------------
     def one_electron(self, center):
@@ -348,6 +354,24 @@ class Context:
         if n < 0:
             check(n)
         return lc.reshape(21, 4)[:n], nq[:n], npr[:n], ms[:n]
+
+    def rhf_gradient(self, P, W, bf_atom):
+        """Electronic part of the RHF nuclear gradient (natom, 3) for the
+        half density P, W = P F P and the atom of each basis function."""
+        N = self.nbf
+        P = np.ascontiguousarray(np.real(P), dtype=np.float64)
+        W = np.ascontiguousarray(np.real(W), dtype=np.float64)
+        at = np.ascontiguousarray(bf_atom, dtype=np.int32)
+        if P.shape != (N, N) or W.shape != (N, N) or at.shape != (N,):
+            raise ValueError("P and W must be N x N and bf_atom of length N")
+        g = np.zeros(3 * self.natom)
+        check(lib.mmd_rhf_gradient(self._h, _ptr(P), _ptr(W), _ptr(at), _ptr(g)))
+        return g.reshape(-1, 3)
+
+    def last_gradient_ms(self):
+        v = _c_double(0.0)
+        check(lib.mmd_last_gradient_ms(self._h, ctypes.byref(v)))
+        return v.value
 
     def fock_profile_sums(self):
         """Sums since profiling was turned on, per class: (l_a, l_b, l_c, l_d),

@@ -12,6 +12,7 @@ import numpy as np
 
 from mmd import _native
 from mmd.integrals.twoe import Basis
+from mmd.forces import Forces
 from mmd.scf import SCF
 
 BOHR_ANGSTROM = 0.52917721092   # mmd/molecule.py:224-226
@@ -59,7 +60,7 @@ class Atom(object):
         self.velocities = np.zeros(3)
 
 
-class Molecule(SCF):
+class Molecule(SCF, Forces):
     """A closed-shell molecule: geometry (Angstrom), basis set and integrals.
 
     Attributes after ``build()``: S, T, V, M, L, Core, X, U, nuc_energy and,
@@ -106,9 +107,6 @@ class Molecule(SCF):
     @property
     def _forces(self):
         return np.concatenate([atom.forces for atom in self.atoms]).reshape(-1, 3)
-
-    def forces(self):
-        raise NotImplementedError("nuclear gradients are not part of this build")
 
     # -- basis -------------------------------------------------------------------
     def formBasis(self):

@@ -575,6 +575,52 @@ def bench_headline(args):
     return out
 
 
+def bench_gradient(args):
+    """RHF nuclear gradient (Molecule.forces, mmd_rhf_gradient) of (H2O)4 /
+    6-31G (N = 52, 5.5 A cube, seed 0, perturbed monomers) after a converged
+    direct SCF; one step = one forces() call (derivative integrals of all
+    atoms and directions, contraction, host transfer).  CPU baseline: the
+    oracle's gradient (forces.py restated, one core) of one water in the
+    same basis, scaled by the canonical basis-function quartet count."""
+    import numpy as np
+    import oracle
+    from mmd.molecule import Molecule
+    from mmd.workloads import water_cluster
+    geom = water_cluster(4, 5.5, seed=0, perturb=True)
+    mol = Molecule(geom, "6-31g")
+    mol.RHF(doPrint=False, direct=True, conver=1e-10)
+    for _ in range(max(1, args.warmup)):
+        mol.forces()
+    t0 = time.perf_counter()
+    dev = []
+    for _ in range(args.steps):
+        f = mol.forces()
+        dev.append(mol.gradient_ms)
+    el = (time.perf_counter() - t0) / args.steps
+    N = mol.nbasis
+    nq = (N * (N + 1) // 2) * (N * (N + 1) // 2 + 1) // 2
+    out = {"metric": "RHF nuclear gradients/s", "value": 1.0 / el, "unit": "gradients/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * el, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "(H2O)4 6-31G RHF nuclear gradient", "n_basis": N, "seed": 0, "box_A": 5.5,
+                      "canonical_quartets": nq},
+           "device_ms_median": float(np.median(dev)), "max_abs_force": float(np.abs(f).max()),
+           "sum_of_forces": float(np.abs(f.sum(0)).max())}
+    if not args.no_cpu_baseline:
+        w = water_cluster(1, 3.0, seed=0, perturb=True)
+        om = oracle.OracleMolecule(w, os.path.join(PKG, "mmd", "basis", "6-31g.gbs"))
+        r = oracle.rhf(om, conver=1e-10)
+        t1 = time.perf_counter()
+        om.rhf_gradient(r["P"], r["F"])
+        tw = time.perf_counter() - t1
+        n1 = om.nbasis * (om.nbasis + 1) // 2
+        nq1 = n1 * (n1 + 1) // 2
+        out["cpu_baseline"] = {"value": 1.0 / (tw * nq / nq1), "unit": "gradients/s", "cores": 1, "kind": "port",
+                               "sample": "oracle gradient of one water / 6-31G (%d quartets, %.2f s), scaled by the "
+                                         "canonical quartet count to %d (extrapolated)" % (nq1, tw, nq)}
+    return out
+
+
 def bench_ao2mo(args):
     """Four-index transformation of the stored ERI tensor of (H2O)4 / cc-pVDZ
     (N = 100, 7 A cube, seed 0) on the GPU: four FP64 GEMM quarter
@@ -669,7 +715,7 @@ def main():
     ap.add_argument("--fp32-below", type=float, default=0.0,
                     help="FP32 tier threshold of the headline's full form (0: all FP64; at these sizes the "
                          "tier's second pass over the units costs more than its FP32 arithmetic saves)")
-    ap.add_argument("--workload", choices=("headline", "h2o8", "h2o16-tier", "batch512", "ao2mo",
+    ap.add_argument("--workload", choices=("headline", "h2o8", "h2o16-tier", "batch512", "ao2mo", "gradient",
                                            "copy-calibration"), default="headline",
                     help="headline: 512 x (H2O)2-6 6-31G, full form vs plain FP64 (default); "
                          "h2o8: (H2O)8 cc-pVDZ FP64; h2o16-tier: (H2O)16 cc-pVDZ FP64 vs FP32 tier; "
@@ -707,6 +753,9 @@ def main():
         return
     if args.workload == "ao2mo":
         print(json.dumps(bench_ao2mo(args)))
+        return
+    if args.workload == "gradient":
+        print(json.dumps(bench_gradient(args)))
         return
 
     geom = water_cluster(8, 7.0, seed=0)

@@ -138,9 +138,10 @@ __device__ inline void tri_decode_g(int64_t idx, int64_t* r, int64_t* c)
     *c = idx - x * (x + 1) / 2;
 }
 
-// Two-electron part: one lane per canonical quartet i>=j, k>=l, ij>=kl
-// (forces.py:62-80); the workgroup sums its lanes' contributions per atom in
-// LDS and adds them to grad once.
+// Two-electron part: one wavefront per canonical quartet i>=j, k>=l, ij>=kl
+// (forces.py:62-80), its primitive quartets spread over the lanes and summed
+// by a wavefront reduction; the workgroup sums the quartets' contributions
+// per atom in LDS and adds them to grad once.
 __global__ void __launch_bounds__(256) grad_eri_kernel(int N, const BFDesc* __restrict__ bfs,
                                                        const double* __restrict__ pexp,
                                                        const double* __restrict__ pw,
@@ -151,11 +152,13 @@ __global__ void __launch_bounds__(256) grad_eri_kernel(int N, const BFDesc* __re
     extern __shared__ double gl[];   // 3 * natom
     for (int i = threadIdx.x; i < 3 * natom; i += blockDim.x) gl[i] = 0.0;
     __syncthreads();
+    const int lane = threadIdx.x & 63;
     const int64_t npair = (int64_t)N * (N + 1) / 2;
     const int64_t total = npair * (npair + 1) / 2;
     const size_t n = (size_t)N;
-    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t wave0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwave = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t idx = wave0; idx < total; idx += nwave) {   // uniform across the wavefront
         int64_t ij, kl, i, j, k, l;
         tri_decode_g(idx, &ij, &kl);
         tri_decode_g(ij, &i, &j);
@@ -177,23 +180,25 @@ __global__ void __launch_bounds__(256) grad_eri_kernel(int N, const BFDesc* __re
         const int la[3] = {fa.l, fa.m, fa.n}, lb[3] = {fb.l, fb.m, fb.n};
         const int lc[3] = {fc.l, fc.m, fc.n}, ld[3] = {fd.l, fd.m, fd.n};
         double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        const int nab = fa.np * fb.np, ncd = fc.np * fd.np;
         GPair Pp, Qp;
-        for (int ia = 0; ia < fa.np; ++ia)
-            for (int ib = 0; ib < fb.np; ++ib) {
-                gpair(fa, fb, ia, ib, pexp, pw, Pp);
-                for (int ic = 0; ic < fc.np; ++ic)
-                    for (int id = 0; id < fd.np; ++id) {
-                        gpair(fc, fd, ic, id, pexp, pw, Qp);
-                        prim_quartet_grad(la, lb, lc, ld, Pp, Qp, boys, g);
-                    }
+        for (int q = lane; q < nab * ncd; q += 64) {
+            const int pab = q / ncd, pcd = q - pab * ncd;
+            gpair(fa, fb, pab / fb.np, pab % fb.np, pexp, pw, Pp);
+            gpair(fc, fd, pcd / fd.np, pcd % fd.np, pexp, pw, Qp);
+            prim_quartet_grad(la, lb, lc, ld, Pp, Qp, boys, g);
+        }
+        for (int off = 32; off > 0; off >>= 1)
+            for (int d = 0; d < 9; ++d) g[d] += __shfl_xor(g[d], off);
+        if (lane == 0) {
+            const int at[4] = {bf_atom[i], bf_atom[j], bf_atom[k], bf_atom[l]};
+            for (int x = 0; x < 3; ++x) {
+                const double dA = g[x], dB = g[3 + x], dC = g[6 + x];
+                atomicAdd(&gl[3 * at[0] + x], wgt * dA);
+                atomicAdd(&gl[3 * at[1] + x], wgt * dB);
+                atomicAdd(&gl[3 * at[2] + x], wgt * dC);
+                atomicAdd(&gl[3 * at[3] + x], -wgt * (dA + dB + dC));
             }
-        const int at[4] = {bf_atom[i], bf_atom[j], bf_atom[k], bf_atom[l]};
-        for (int x = 0; x < 3; ++x) {
-            const double dA = g[x], dB = g[3 + x], dC = g[6 + x];
-            atomicAdd(&gl[3 * at[0] + x], wgt * dA);
-            atomicAdd(&gl[3 * at[1] + x], wgt * dB);
-            atomicAdd(&gl[3 * at[2] + x], wgt * dC);
-            atomicAdd(&gl[3 * at[3] + x], -wgt * (dA + dB + dC));
         }
     }
     __syncthreads();
@@ -331,8 +336,8 @@ hipError_t launch_rhf_gradient(int nbf, const BFDesc* bfs, const double* pexp, c
                        pw, natom, atoms, boys, bf_atom, P, W, grad);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const int64_t total = npair * (npair + 1) / 2;
-    const int64_t grid = std::min<int64_t>(std::max<int64_t>(1, (total + 255) / 256), 8192);
+    const int64_t total = npair * (npair + 1) / 2;   // quartets: one wavefront each, four per workgroup
+    const int64_t grid = std::min<int64_t>(std::max<int64_t>(1, (total + 3) / 4), 16384);
     hipLaunchKernelGGL(grad_eri_kernel, dim3((unsigned)grid), dim3(256), sizeof(double) * 3 * natom, st, nbf, bfs,
                        pexp, pw, boys, bf_atom, natom, P, grad);
     return hipGetLastError();

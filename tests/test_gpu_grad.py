@@ -5,9 +5,9 @@ restatement of mmd/forces.py and cython/grad.pyx, itself pinned by
 tests/test_oracle.py against finite differences).  Tolerances: the
 reference test's own for H2 (assert_allclose atol 1e-12); 1e-10 against
 the oracle on identical P and F; invariances 1e-10; central differences of
-the GPU energy (h = 1e-4 bohr, SCF to ||dP|| < 1e-12) 3e-7: the energy
-expression is first order in the density error, which the difference
-quotient divides by 2h."""
+the GPU energy (h = 1e-4 bohr) 1e-7, with the SCFs of the differences
+unscreened (acc2e = 0: the screen's decisions change with the geometry and
+their 1e-12-sized steps, divided by 2h, would dominate the quotient)."""
 import os
 
 import numpy as np
@@ -64,7 +64,7 @@ def test_gradient_cluster_invariance_and_differences(native):
     from mmd.molecule import Molecule
     geom = water_cluster(2, 5.0, seed=3, perturb=True)
     mol = Molecule(geom, "6-31g")
-    mol.RHF(doPrint=False, direct=True, conver=1e-12)
+    mol.RHF(doPrint=False, direct=True, conver=1e-10, acc2e=0.0)
     f = mol.forces()
     xyz = np.array([a.origin for a in mol.atoms])
     print("(H2O)2 6-31G: sum %s torque %s, device ms %.2f" % (f.sum(0), np.cross(xyz, f).sum(0), mol.gradient_ms))
@@ -81,8 +81,8 @@ def test_gradient_cluster_invariance_and_differences(native):
             vals[x] += sgn * h * 0.52917721092
             lines[1 + a] = parts[0] + " " + " ".join("%.12f" % v for v in vals)
             m = Molecule("\n".join(lines) + "\n", "6-31g")
-            m.RHF(doPrint=False, direct=True, conver=1e-12)
+            m.RHF(doPrint=False, direct=True, conver=1e-10, acc2e=0.0)
             e.append(m.energy.real)
         fd = (e[0] - e[1]) / (2 * h)
         print("atom %d axis %d: analytic %.10f, differences %.10f" % (a, x, -f[a, x], fd))
-        assert abs(-f[a, x] - fd) < 3e-7
+        assert abs(-f[a, x] - fd) < 1e-7

@@ -540,8 +540,10 @@ def bench_headline(args):
         "warmup": args.warmup, "ms_per_step": 1e3 * el_full, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
         "config": {"workload": "%d x (H2O)n, n=2..6, 6-31G direct RHF, full form (batched on the device in "
-                               "groups of %d, %d groups in flight, HIP-graph iterations, learned start, FP32 "
-                               "tier below %g) vs plain FP64 one at a time" % (len(geoms), G, args.batch_streams, args.fp32_below),
+                               "groups of %d, %d groups in flight, HIP-graph iterations, learned start, %s) vs "
+                               "plain FP64 one at a time" % (len(geoms), G, args.batch_streams,
+                                                            "FP32 tier below %g" % args.fp32_below
+                                                            if args.fp32_below > 0 else "FP64"),
                    "molecules": len(geoms), "seed": 1, "basis": "6-31g", "group": G,
                    "groups_in_flight": args.batch_streams, "class_streams": args.class_streams,
                    "fp32_below": args.fp32_below, "conver": 1e-8,

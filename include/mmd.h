@@ -192,6 +192,17 @@ int mmd_last_gradient_ms(mmd_ctx *ctx, double *ms);
  * mmd_fock_stats fills up to 21 entries (lclass: 4 per class) and returns
  * the number of classes.
  */
+/*
+ * Kernel choice for the six shell-quartet classes that have both a row
+ * kernel and a matrix-core kernel ((dd|dp), (dd|pp), (dd|ds), (dp|dp),
+ * (dp|pp), (dp|ds)); no counterpart in the reference (tuning).  mode 0: row
+ * kernels; 1: matrix cores; 2 (default): the first two measured symmetrised
+ * builds after a pair table is built time one variant each, side by side
+ * with the other classes as usual, and the faster is kept per class; -1
+ * leaves the mode as it is.  choice (optional, 21 entries in class order):
+ * per class 0 = row kernel, 1 = matrix cores, -1 = not decided yet.
+ */
+int mmd_set_class_impl(mmd_ctx *ctx, int mode, int32_t *choice);
 int mmd_set_profiling(mmd_ctx *ctx, int on);
 int mmd_fock_stats(mmd_ctx *ctx, int32_t *lclass, int64_t *nquart, int64_t *nprim, double *ms);
 /*

@@ -210,6 +210,16 @@ struct mmd_ctx {
     double plan_ms[NQCLASS] = {0};
     int plan_countdown = 0;
     bool plan_measuring = false;
+    // The six classes with both a row kernel and a matrix-core kernel:
+    // impl_mode 0 takes the row kernels, 1 the matrix cores, 2 times both on
+    // the first two measured builds of a pair table (the classes running side
+    // by side as usual) and keeps the faster per class (impl_choice: -1
+    // undecided, 0 row, 1 matrix cores).
+    int impl_mode = 2;
+    int impl_choice[NQCLASS];
+    int tune_phase = 0;
+    double tune_ms[2][NQCLASS] = {};
+    bool tune_measuring = false;
     DevBuf<unsigned long long> d_nprim;   // NPRIM_SLOTS counters (FockScratch::nprim)
     DevBuf<int> d_rowctr;
     // Work units of the row-kernel classes (built once per pair table):
@@ -501,8 +511,12 @@ int build_pairs(mmd_ctx* c, const DiagView* diag)
     HIP_TRY(c->d_count.reserve(2 * NQCLASS));
     c->pairs_ok = true;
     c->units_ok = false;
-    for (int qc = 0; qc < NQCLASS; ++qc) c->plan_ms[qc] = 0.0;
+    for (int qc = 0; qc < NQCLASS; ++qc) {
+        c->plan_ms[qc] = 0.0;
+        c->impl_choice[qc] = -1;
+    }
     c->plan_countdown = 0;
+    c->tune_phase = 0;
     return MMD_OK;
 }
 
@@ -628,18 +642,33 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol, FockScratch& W0)
     }
     bool row_kernel[NQCLASS], kj_lds[NQCLASS];
     int maxpp[NQCLASS];
+    // a build that times the row or the matrix-core variant of the classes
+    // that have both, while some of them are undecided
+    bool tuning = false;
+    if (c->impl_mode == 2 && W0.allow_measure && c->profiling == 0)
+        for (int qc = 0; qc < NQCLASS; ++qc) {
+            int la, lb, lc, ld;
+            qclass_decode(qc, &la, &lb, &lc, &ld);
+            if (c->qc_total[qc] && fock_class_has_both(la, lb, lc, ld) && c->impl_choice[qc] < 0) tuning = true;
+        }
     for (int qc = 0; qc < NQCLASS; ++qc) {
         int la, lb, lc, ld, pcb, pck;
         qclass_decode(qc, &la, &lb, &lc, &ld);
         class_of(qc, &pcb, &pck);
         maxpp[qc] = 1;
         for (int P = c->pc_off[pcb]; P < c->pc_off[pcb + 1]; ++P) maxpp[qc] = std::max(maxpp[qc], c->pairs[P].npp);
+        bool mfma = fock_class_is_wave(la, lb, lc, ld);
+        if (mfma && fock_class_has_both(la, lb, lc, ld)) {
+            if (c->impl_mode == 0) mfma = false;
+            else if (c->impl_mode == 2 && c->impl_choice[qc] >= 0) mfma = c->impl_choice[qc] == 1;
+            else if (c->impl_mode == 2 && tuning) mfma = c->tune_phase == 1;
+        }
         // the ket-side Coulomb buffer where it fits; the row kernel without it
         // where only that fits; else the wavefront-per-quartet kernel
         kj_lds[qc] = fock_class_lds(qc, 0, c->rw, maxpp[qc], 1) <= c->max_lds;
-        row_kernel[qc] = !fock_class_is_wave(la, lb, lc, ld) &&
-                         (kj_lds[qc] || fock_class_lds(qc, 0, c->rw, maxpp[qc], 0) <= c->max_lds);
+        row_kernel[qc] = !mfma && (kj_lds[qc] || fock_class_lds(qc, 0, c->rw, maxpp[qc], 0) <= c->max_lds);
     }
+    c->tune_measuring = tuning;
     {
         // survivor lists only for the classes that use them, packed; with the
         // FP32 tier each class has a second list right after the first
@@ -685,7 +714,7 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol, FockScratch& W0)
     // misses the latency floor of the small classes (150-300 us each at
     // (H2O)8/cc-pVDZ, whatever their work), which left the longest stream
     // twice as long as the others.  One build in PLAN_EVERY measures again.
-    const bool measure = W0.allow_measure && c->profiling == 0 && nstream > 1 && c->plan_countdown <= 0;
+    const bool measure = (W0.allow_measure && c->profiling == 0 && nstream > 1 && c->plan_countdown <= 0) || tuning;
     bool measured = nord > 0;
     for (int oi = 0; oi < nord; ++oi) measured = measured && c->plan_ms[order[oi]] > 0.0;
     if (measured)
@@ -896,6 +925,7 @@ int mmd_ctx_create(mmd_ctx** out, int device_id)
     mmd_ctx* c = new (std::nothrow) mmd_ctx();
     if (!c) return fail(MMD_ERR_NOMEM, "context allocation failed");
     c->device = device_id;
+    for (int qc = 0; qc < NQCLASS; ++qc) c->impl_choice[qc] = -1;
     // Classes on more streams than the process has hardware queues would
     // share queues in an order the longest-first plan does not see: spread
     // them over min(8, GPU_MAX_HW_QUEUES) streams (HIP's default is 4).  The
@@ -1370,6 +1400,24 @@ int mmd_fock_delta(mmd_ctx* ctx, const double* dP_re, const double* dP_im, doubl
     } else if (!raw) {
         --ctx->plan_countdown;
     }
+    if (!raw && ctx->tune_measuring) {
+        // a build of the implementation choice: record, and decide after both
+        for (int qc = 0; qc < NQCLASS; ++qc) {
+            float cms = 0.f;
+            if (ctx->qc_total[qc]) HIP_TRY(hipEventElapsedTime(&cms, ctx->evc[3 * qc], ctx->evc[3 * qc + 2]));
+            ctx->tune_ms[ctx->tune_phase][qc] = cms;
+        }
+        ctx->tune_measuring = false;
+        if (++ctx->tune_phase == 2) {
+            for (int qc = 0; qc < NQCLASS; ++qc) {
+                int la, lb, lc, ld;
+                qclass_decode(qc, &la, &lb, &lc, &ld);
+                if (fock_class_has_both(la, lb, lc, ld))
+                    ctx->impl_choice[qc] = ctx->tune_ms[1][qc] < ctx->tune_ms[0][qc] ? 1 : 0;
+            }
+            ctx->plan_countdown = 0;   // plan again on the chosen kernels
+        }
+    }
     if (ctx->profiling && !raw) return collect_profile(ctx);
     return MMD_OK;
 }
@@ -1542,6 +1590,27 @@ int mmd_fock_profile_sums(mmd_ctx* ctx, int32_t* lclass, int64_t* nquart, int64_
         if (builds) builds[qc] = a.builds;
     }
     return NQCLASS;
+}
+
+int mmd_set_class_impl(mmd_ctx* ctx, int mode, int32_t* choice)
+{
+    if (!ctx) return fail(MMD_ERR_ARG, "null context");
+    if (mode < -1 || mode > 2) return fail(MMD_ERR_ARG, "mode must be 0 (row), 1 (matrix cores), 2 (measured) or -1");
+    if (mode >= 0 && mode != ctx->impl_mode) {
+        ctx->impl_mode = mode;
+        for (int qc = 0; qc < NQCLASS; ++qc) ctx->impl_choice[qc] = -1;
+        ctx->tune_phase = 0;
+        ctx->plan_countdown = 0;
+    }
+    if (choice)
+        for (int qc = 0; qc < NQCLASS; ++qc) {
+            int la, lb, lc, ld;
+            qclass_decode(qc, &la, &lb, &lc, &ld);
+            const bool both = fock_class_has_both(la, lb, lc, ld);
+            const bool mf = fock_class_is_wave(la, lb, lc, ld);
+            choice[qc] = !mf ? 0 : (!both ? 1 : (ctx->impl_mode == 2 ? ctx->impl_choice[qc] : ctx->impl_mode));
+        }
+    return MMD_OK;
 }
 
 int mmd_set_precision(mmd_ctx* ctx, double fp32_below)

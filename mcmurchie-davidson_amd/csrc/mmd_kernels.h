@@ -47,8 +47,13 @@ inline bool fock_class_is_mfma(int la, int lb, int lc, int ld)
     return la == 2 && ((lb == 2 && lc + ld >= 2) || (lb == 1 && lc == 2) || (lb == 1 && lc == 1 && ld == 1));
 }
 // Classes served by a wavefront-per-quartet kernel (survivor lists) instead of
-// the unrolled row kernel.
+// the unrolled row kernel by default.
 inline bool fock_class_is_wave(int la, int lb, int lc, int ld) { return fock_class_is_mfma(la, lb, lc, ld); }
+// Matrix-core classes that also have a compiled row kernel (all but (dd|dd)).
+inline bool fock_class_has_both(int la, int lb, int lc, int ld)
+{
+    return fock_class_is_mfma(la, lb, lc, ld) && !(la == 2 && lb == 2 && lc == 2 && ld == 2);
+}
 
 // A single Cartesian basis function for the generic (per basis-function
 // quartet) kernel: centre, exponents (l,m,n), primitive range; the

@@ -84,6 +84,8 @@ lib.mmd_set_profiling.restype = _c_int
 lib.mmd_set_profiling.argtypes = [_ctx_p, _c_int]
 lib.mmd_fock_stats.restype = _c_int
 lib.mmd_fock_stats.argtypes = [_ctx_p, _vp, _vp, _vp, _vp]
+lib.mmd_set_class_impl.restype = _c_int
+lib.mmd_set_class_impl.argtypes = [_ctx_p, _c_int, _vp]
 lib.mmd_rhf_gradient.restype = _c_int
 lib.mmd_rhf_gradient.argtypes = [_ctx_p, _vp, _vp, _vp, _vp]
 lib.mmd_last_gradient_ms.restype = _c_int
@@ -129,7 +131,7 @@ EXPORTED = [
     "mmd_last_error", "mmd_version", "mmd_device_count", "mmd_ctx_create", "mmd_ctx_destroy",
     "mmd_set_basis", "mmd_set_atoms", "mmd_one_electron", "mmd_schwarz", "mmd_set_schwarz",
     "mmd_eri_tensor", "mmd_eri_list", "mmd_fock_delta", "mmd_fock_delta_noalloc", "mmd_jk_tensor", "mmd_boys",
-    "mmd_last_fock_ms", "mmd_set_profiling", "mmd_fock_stats", "mmd_fock_profile_sums", "mmd_rhf_gradient",
+    "mmd_last_fock_ms", "mmd_set_profiling", "mmd_fock_stats", "mmd_fock_profile_sums", "mmd_rhf_gradient", "mmd_set_class_impl",
     "mmd_last_gradient_ms", "mmd_set_precision", "mmd_fock_tiers",
     "mmd_set_streams", "mmd_set_lds_limit", "mmd_overlap_powers", "mmd_dgemm", "mmd_set_molecules",
     "mmd_batch_rhf", "mmd_scf_begin", "mmd_scf_step", "mmd_scf_density", "mmd_scf_get", "mmd_ao2mo",
@@ -354,6 +356,15 @@ class Context:
         if n < 0:
             check(n)
         return lc.reshape(21, 4)[:n], nq[:n], npr[:n], ms[:n]
+
+    def set_class_impl(self, mode=None):
+        """Kernel choice of the classes with both a row and a matrix-core
+        kernel: 0 row, 1 matrix cores, 2 measured per class (default); None
+        keeps the mode.  Returns the per-class choice (21 entries: 0 row,
+        1 matrix cores, -1 undecided)."""
+        ch = np.zeros(21, dtype=np.int32)
+        check(lib.mmd_set_class_impl(self._h, -1 if mode is None else int(mode), _ptr(ch)))
+        return ch
 
     def rhf_gradient(self, P, W, bf_atom):
         """Electronic part of the RHF nuclear gradient (natom, 3) for the

@@ -118,6 +118,33 @@ def test_fock_lds_fallbacks_vs_generic(native, n, seed):
         ctx.set_lds_limit(0)
 
 
+def test_fock_class_impl_modes_vs_generic(native):
+    """The six classes with both kernels give the generic kernel's G on the
+    row kernels (mode 0), on the matrix cores (mode 1) and with the measured
+    choice (mode 2, over the builds that time each variant and after)."""
+    from mmd.molecule import Molecule
+    mol = Molecule(water_cluster(3, 7.0, 2), "cc-pvdz")
+    mol.build(direct=True)
+    ctx = mol.native
+    rng = np.random.default_rng(11)
+    dP = _sym_random(mol.nbasis, rng)
+    Gr, _, _ = ctx.fock_delta(dP, None, tol=1e-12, raw=True)
+    Gr = 0.5 * (Gr + Gr.T)
+    try:
+        for mode, builds in ((0, 1), (1, 1), (2, 4)):
+            ctx.set_class_impl(mode)
+            for b in range(builds):
+                G, _, _ = ctx.fock_delta(dP, None, tol=1e-12, raw=False)
+                err = np.max(np.abs(G - Gr))
+                print("mode %d build %d: max |G - G_generic| %.2e, choice %s" % (mode, b, err,
+                                                                              ctx.set_class_impl().tolist()))
+                assert err < 1e-10
+        ch = ctx.set_class_impl()
+        assert all(c in (0, 1) for c in ch)
+    finally:
+        ctx.set_class_impl(2)
+
+
 def test_fock_h2o8_classes_vs_generic(native):
     """The symmetrised class kernels at N = 200 against the per-function kernel
     (formPT's loop, symmetrised) on a density of the size of an SCF update."""

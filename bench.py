@@ -70,9 +70,11 @@ def class_flops(la, lb, lc, ld):
     return 2 * H(Lab) * H(Lcd) * Ccd + 2 * Cab * H(Lab) * Ccd + 3 * NR(Lab + Lcd) + 40
 
 
-def run_scf(geom, basis, profile=False, serial=False):
+def run_scf(geom, basis, profile=False, serial=False, class_impl=None):
     from mmd.molecule import Molecule
     mol = Molecule(geom, basis)
+    if class_impl is not None:
+        mol.native.set_class_impl(class_impl)
     if profile:
         ctx = mol.native
         ctx.set_profiling(True, serial=serial)
@@ -156,6 +158,8 @@ def bench_tier(args):
     def run(fp32_below):
         mol = Molecule(geom, "cc-pvdz")
         ctx = mol.native
+        if args.class_impl is not None:
+            ctx.set_class_impl(args.class_impl)
         ctx.set_precision(fp32_below)
         tiers = []
         orig_delta, orig_step = ctx.fock_delta, ctx.scf_step
@@ -722,6 +726,9 @@ def main():
                     help="headline: 512 x (H2O)2-6 6-31G, full form vs plain FP64 (default); "
                          "h2o8: (H2O)8 cc-pVDZ FP64; h2o16-tier: (H2O)16 cc-pVDZ FP64 vs FP32 tier; "
                          "batch512: 512 small 6-31G clusters through worker processes")
+    ap.add_argument("--class-impl", type=int, choices=(0, 1, 2), default=None,
+                    help="h2o8 / h2o16-tier: kernels of the classes with a row and a matrix-core kernel "
+                         "(0 row, 1 matrix cores, 2 measured per class, the default)")
     ap.add_argument("--steps-only", action="store_true",
                     help="headline: run the full-form steps only (for counter and trace runs; no value)")
     ap.add_argument("--pmc-summary", nargs="+", metavar="CSV",
@@ -763,7 +770,7 @@ def main():
     geom = water_cluster(8, 7.0, seed=0)
     basis = "cc-pvdz"
     for _ in range(args.warmup):
-        run_scf(geom, basis)
+        run_scf(geom, basis, class_impl=args.class_impl)
     print("warmup done", file=sys.stderr, flush=True)
 
     mols = []
@@ -772,7 +779,7 @@ def main():
         # per-class HIP events on the classes' own streams, inside the timed
         # region (last step)
         mols.append(run_scf(geom, basis, profile=(not args.no_profile) and s == args.steps - 1,
-                            serial=args.serial_classes))
+                            serial=args.serial_classes, class_impl=args.class_impl))
         print("step %d done" % s, file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
     scf_per_s = args.steps / elapsed
@@ -789,7 +796,9 @@ def main():
         "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "(H2O)8 cc-pVDZ direct FP64 RHF", "n_basis": N, "seed": 0, "box_A": 7.0,
-                   "min_OO_A": 2.7, "acc2e": 1e-12, "conver": 1e-8, "hw_queues": int(HW_QUEUES)},
+                   "min_OO_A": 2.7, "acc2e": 1e-12, "conver": 1e-8, "hw_queues": int(HW_QUEUES),
+                   "class_impl": 2 if args.class_impl is None else args.class_impl},
+        "class_choice": [int(x) for x in m0.native.set_class_impl()],
         "fock_build_ms": fock_med, "quartets_per_s": nquart / (fock_med * 1e-3),
         "unique_bf_quartets": nquart, "scf_iterations": int(m0.iterations), "energy": energies[-1],
         "delta_energy": float(abs(m0.delta_energy.real)),

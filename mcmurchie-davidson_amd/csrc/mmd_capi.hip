@@ -645,7 +645,7 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol, FockScratch& W0)
     // a build that times the row or the matrix-core variant of the classes
     // that have both, while some of them are undecided
     bool tuning = false;
-    if (c->impl_mode == 2 && W0.allow_measure && c->profiling == 0)
+    if (c->impl_mode == 2 && W0.allow_measure)
         for (int qc = 0; qc < NQCLASS; ++qc) {
             int la, lb, lc, ld;
             qclass_decode(qc, &la, &lb, &lc, &ld);

@@ -193,6 +193,7 @@ struct mmd_ctx {
     DevBuf<int2> d_mol_atoms;
     DevBuf<int2> d_plist;
     DevBuf<int4> d_mol_blocks;          // per molecule: first function, functions, shells [s0, s1)
+    DevBuf<double> d_mol_dmax;          // per molecule: max |dP| of the build
     int64_t npl = 0;
     int rw = 0;
     // per pair class, per molecule: the class-local range [first, end) of its pairs
@@ -366,6 +367,7 @@ int molecule_shell_ranges(mmd_ctx* c)
         for (int m = 0; m < c->nmol; ++m)
             blk[m] = make_int4(c->mol_f0[m], c->mol_n[m], c->mol_s0[m], c->mol_s0[m + 1]);
         HIP_TRY(c->d_mol_blocks.reserve(c->nmol));
+        HIP_TRY(c->d_mol_dmax.reserve(c->nmol));
         HIP_TRY(hipMemcpyAsync(c->d_mol_blocks.p, blk.data(), sizeof(int4) * c->nmol, hipMemcpyHostToDevice, c->st));
     }
     return MMD_OK;
@@ -622,6 +624,7 @@ struct FockScratch {
     // the row kernels (FP64, FP32 tier), primitive quartets (FP32 tier)
     unsigned long long* nprim = nullptr;
     double* Dsh = nullptr;                 // nshell^2
+    const double* mol_dmax = nullptr;      // batches: per-molecule max |dP| (units of molecules at 0 skipped)
     int2* surv = nullptr;
     int64_t surv_off[NQCLASS] = {0};       // survivor lists, packed: wave classes only
     int64_t surv_cap = -1;                 // entries available at surv; < 0: grow the context's buffer
@@ -755,6 +758,7 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol, FockScratch& W0)
             R.tri = pcb == pck ? 1 : 0;
             R.nshell = (int)c->shells.size();
             R.Dsh = W0.Dsh;
+            R.mol_dmax = W0.mol_dmax;
             R.row_counter = W0.rowctr + qc;
             R.maxpp = maxpp[qc];
             R.kj = kj_lds[qc] ? 1 : 0;
@@ -1710,7 +1714,7 @@ int enqueue_fock_sym(mmd_ctx* c, const double* dP, double tol, double* Gout)
     const bool blocks = c->nmol > 1;   // a batch: only the molecules' diagonal blocks are live
     if (blocks) {
         HIP_TRY(launch_block_prep(c->nmol, c->d_mol_blocks.p, N, dP, c->d_absD.p, c->d_G.p, c->d_sh_bf0.p,
-                                  c->d_sh_nbf.p, (int)c->shells.size(), c->d_Dsh.p, c->st));
+                                  c->d_sh_nbf.p, (int)c->shells.size(), c->d_Dsh.p, c->d_mol_dmax.p, c->st));
     } else {
         HIP_TRY(launch_abs_density((int)nn, dP, nullptr, c->d_absD.p, c->st));
         HIP_TRY(launch_shell_block_max((int)c->shells.size(), c->d_sh_bf0.p, c->d_sh_nbf.p, N, c->d_absD.p,
@@ -1729,6 +1733,7 @@ int enqueue_fock_sym(mmd_ctx* c, const double* dP, double tol, double* Gout)
     W.nprim = c->d_nprim.p;
     W.profile_events = true;
     W.Dsh = c->d_Dsh.p;
+    W.mol_dmax = blocks ? c->d_mol_dmax.p : nullptr;
     W.surv = c->d_surv.p;
     W.surv_cap = (int64_t)c->d_surv.n;   // reserved by the caller
     W.allow_measure = false;

@@ -467,6 +467,9 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
         const int segn = U.w - U.z;
         const int bnp = ip1u - ip0u;   // bra primitive pairs of the unit (<= PSLICE)
         const ShellPair bra = A.pairs[R.boff + P];
+        // a molecule of a batch whose density no longer changes (converged)
+        // contributes nothing: its quartets would all fail the screen
+        if (R.mol_dmax && R.mol_dmax[bra.mol] == 0.0) continue;
         const int nrows = bra.same ? S::NA : S::NA + S::NB;
         const int c0 = bra.c0;       // the rows hold the molecule's columns c0 .. c0+RW-1
         double* rowA = rows - c0;

@@ -92,10 +92,11 @@ hipError_t launch_rhf_gradient(int nbf, const BFDesc* bfs, const double* pexp, c
                                const double* atoms, const double* boys, const int* bf_atom, const double* P,
                                const double* W, double* grad, hipStream_t st);
 // Block-diagonal batches (blocks: per molecule {first function, functions,
-// first shell, end shell}): |dP|, zero G and shell maxima in the blocks only;
-// the symmetrisation of the blocks.
+// first shell, end shell}): |dP|, zero G, shell maxima and each molecule's
+// max |dP| (mol_dmax) in the blocks only; the symmetrisation of the blocks.
 hipError_t launch_block_prep(int nmol, const int4* blocks, int N, const double* dP, double* absD, double* G,
-                             const int* sh_bf0, const int* sh_nbf, int nshell, double* Dsh, hipStream_t st);
+                             const int* sh_bf0, const int* sh_nbf, int nshell, double* Dsh, double* mol_dmax,
+                             hipStream_t st);
 hipError_t launch_block_symmetrize(int nmol, const int4* blocks, int N, const double* G, double* out,
                                    hipStream_t st);
 hipError_t launch_shell_block_max(int nshell, const int* sh_bf0, const int* sh_nbf, int N, const double* absD,

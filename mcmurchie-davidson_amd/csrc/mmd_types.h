@@ -64,6 +64,7 @@ struct RowArgs {
     int kj;                     // LDS holds the ket-side Coulomb buffer (classes that have one)
     int nshell;
     const double* Dsh;          // shell-block maxima of |dP| (nshell x nshell)
+    const double* mol_dmax;     // batches: max |dP| of each molecule, whose units are skipped at 0; or null
     double tier_lo, tier_hi;    // this launch takes quartets with tier_lo <= bound < tier_hi
     int* row_counter;           // next work unit (zeroed before the launch)
     unsigned long long* stat_nquart;   // optional statistics

@@ -929,16 +929,23 @@ __global__ void __launch_bounds__(256) block_prep_kernel(const int4* __restrict_
                                                          const double* __restrict__ dP, double* __restrict__ absD,
                                                          double* __restrict__ G, const int* __restrict__ bf0,
                                                          const int* __restrict__ nbf, int nshell,
-                                                         double* __restrict__ Dsh)
+                                                         double* __restrict__ Dsh, double* __restrict__ mol_dmax)
 {
+    __shared__ double red[4];
     const int4 b = blocks[blockIdx.x];
     const int f0 = b.x, n = b.y, s0 = b.z, ns = b.w - b.z;
+    double mx = 0.0;
     for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
         const size_t o = (size_t)(f0 + e / n) * N + f0 + e % n;
-        absD[o] = fabs(dP[o]);
+        const double a = fabs(dP[o]);
+        absD[o] = a;
+        mx = fmax(mx, a);
         G[o] = 0.0;
     }
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
     __syncthreads();
+    if (threadIdx.x == 0) mol_dmax[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
     for (int e = threadIdx.x; e < ns * ns; e += blockDim.x) {
         const int s1 = s0 + e / ns, s2 = s0 + e % ns;
         double m = 0.0;
@@ -949,11 +956,12 @@ __global__ void __launch_bounds__(256) block_prep_kernel(const int4* __restrict_
 }
 
 hipError_t launch_block_prep(int nmol, const int4* blocks, int N, const double* dP, double* absD, double* G,
-                             const int* sh_bf0, const int* sh_nbf, int nshell, double* Dsh, hipStream_t st)
+                             const int* sh_bf0, const int* sh_nbf, int nshell, double* Dsh, double* mol_dmax,
+                             hipStream_t st)
 {
     if (nmol <= 0) return hipSuccess;
     hipLaunchKernelGGL(block_prep_kernel, dim3(nmol), dim3(256), 0, st, blocks, N, dP, absD, G, sh_bf0, sh_nbf,
-                       nshell, Dsh);
+                       nshell, Dsh, mol_dmax);
     return hipGetLastError();
 }
 

@@ -343,7 +343,7 @@ def _device_contexts(device, count):
 
 def run_batch_device(geometries, basis="6-31g", group=64, guess="core", fp32_below=0.0, conver=1e-8,
                      acc2e=1e-12, device=0, use_graph=True, keep_density=False, guess_model=None, ctx=None,
-                     streams=2, class_streams=None):
+                     streams=2, class_streams=None, sort_by_size=True):
     """Direct RHF energies of all geometries, `group` molecules at a time in
     one block-diagonal basis on the GPU (mmd_batch_rhf).  guess: 'core' (the
     reference's), 'sad' or 'learned' (initial densities formed for the whole
@@ -352,7 +352,10 @@ def run_batch_device(geometries, basis="6-31g", group=64, guess="core", fp32_bel
     own context (stream), so that one group's host-side set-up overlaps
     another's device work; with ctx given, that context alone.
     class_streams: streams over which each context spreads the classes of
-    a Fock build (default: the context's setting).  Returns one
+    a Fock build (default: the context's setting).  sort_by_size: group the
+    molecules in order of their atom counts (a group iterates until its
+    slowest molecule has converged and steps at the pace of its largest,
+    so groups of similar molecules waste less).  Returns one
     dict per geometry: energy, iterations, converged, nbasis (and the density
     P with keep_density)."""
     if group < 1:
@@ -362,7 +365,11 @@ def run_batch_device(geometries, basis="6-31g", group=64, guess="core", fp32_bel
     if guess == "learned":   # the network's weights once, before any worker needs them
         from mmd.learned import default_model
         guess_model = guess_model or default_model()
-    groups = [list(range(g0, min(len(geometries), g0 + group))) for g0 in range(0, len(geometries), group)]
+    order = list(range(len(geometries)))
+    if sort_by_size:
+        natoms = [sum(1 for ln in g.split("\n")[1:] if ln.strip()) for g in geometries]
+        order.sort(key=lambda i: natoms[i])
+    groups = [order[g0:g0 + group] for g0 in range(0, len(order), group)]
     ctxs = [ctx] if ctx is not None else _device_contexts(device, max(1, min(int(streams), len(groups))))
     if class_streams is not None:
         for c in ctxs:

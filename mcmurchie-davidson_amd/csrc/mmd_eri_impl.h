@@ -409,6 +409,10 @@ __device__ __forceinline__ void static_switch(int i, F&& f)
 // whose bra rows do not fit in LDS to the wavefront-per-quartet kernel.)
 // ---------------------------------------------------------------------------
 constexpr int CLASS_BLOCK = 256;           // threads per workgroup
+#ifndef MMD_BOYS_LDS_MIN_L
+#define MMD_BOYS_LDS_MIN_L 0
+#endif
+constexpr int BOYS_LDS_MIN_L = MMD_BOYS_LDS_MIN_L;   // classes with L below it read Boys rows from L1
 
 template <int QB>
 struct ClassList {
@@ -430,16 +434,25 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
     // ket-side Coulomb buffer (KJ classes, when the launch has room for it)
     const bool kj = S::KJ && R.kj;
     const int nbuf = S::IBUF_DOUBLES + (kj ? S::KJ_DOUBLES : 0);
+    // the class's Boys rows in LDS, or read from the global table (L1) for
+    // the classes below BOYS_LDS_MIN_L, which leaves their workgroups the LDS
+    // for more of them per CU (the default keeps every class in LDS: reading
+    // the rows through L1 made the (ps|ss) class 20% slower)
+    constexpr bool boys_lds = S::L >= BOYS_LDS_MIN_L;
     T* lds_boys = reinterpret_cast<T*>(ibuf + nbuf);
-    T* bp = lds_boys + BOYS_TAYLOR * BOYS_NT;
+    T* bp = boys_lds ? lds_boys + BOYS_TAYLOR * BOYS_NT : lds_boys;
     int* qlist = reinterpret_cast<int*>(bp + (size_t)(R.maxpp < PSLICE ? R.maxpp : PSLICE) * S::BPS);
     int* misc = qlist + ClassList<QB>::QLIST;
     int* pre = misc + 4;         // IBUF classes: prefix sums of the batch's ket slice counts
     int* wsum = pre + QB + 1;    // per-wavefront partial sums of that scan
-    if constexpr (std::is_same<T, float>::value)
-        boys_stage_lds<S::L, float>(lds_boys, A.boysf);
-    else
-        boys_stage_lds<S::L, double>(lds_boys, A.boys);
+    const T* boys_tab;
+    if constexpr (std::is_same<T, float>::value) {
+        if constexpr (boys_lds) boys_stage_lds<S::L, float>(lds_boys, A.boysf);
+        boys_tab = boys_lds ? lds_boys : A.boysf + (size_t)S::L * BOYS_NT;
+    } else {
+        if constexpr (boys_lds) boys_stage_lds<S::L, double>(lds_boys, A.boys);
+        boys_tab = boys_lds ? lds_boys : A.boys + (size_t)S::L * BOYS_NT;
+    }
     for (int i = threadIdx.x; i < nbuf; i += blockDim.x) ibuf[i] = 0.0;
     const int N = A.N;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -559,7 +572,7 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
                         constexpr int C = decltype(ic)::value;
                         constexpr int A0 = C * S::CH, A1 = (C + 1) * S::CH;
                         T acc[(A1 - A0) * S::NCD];
-                        quartet_compute<LA, LB, LC, LD, A0, A1, T>(A, lds_boys, bp, ket, ip0, ip1, jq0, jq1, acc);
+                        quartet_compute<LA, LB, LC, LD, A0, A1, T>(A, boys_tab, bp, ket, ip0, ip1, jq0, jq1, acc);
                         if constexpr (S::IBUF) {
                             double* dst = ibuf + (size_t)idx * S::NABCD + A0 * S::NCD;
                             sfor<(A1 - A0) * S::NCD>(
@@ -736,7 +749,8 @@ inline size_t fock_class_lds_bytes(int RW, int maxpp, int kj)
 {
     using S = ClassShape<LA, LB, LC, LD>;
     return sizeof(double) * ((size_t)(S::NA + S::NB) * RW + S::IBUF_DOUBLES + (kj ? S::KJ_DOUBLES : 0))
-         + sizeof(T) * (BOYS_TAYLOR * BOYS_NT + (size_t)(maxpp < PSLICE ? maxpp : PSLICE) * S::BPS)
+         + sizeof(T) * ((S::L >= BOYS_LDS_MIN_L ? BOYS_TAYLOR * BOYS_NT : 0)
+                        + (size_t)(maxpp < PSLICE ? maxpp : PSLICE) * S::BPS)
          + sizeof(int) * (ClassList<S::QB>::QLIST + 4 + S::QB + 1 + CLASS_BLOCK / 64);
 }
 

@@ -75,3 +75,36 @@ def test_batch_full_form_within_microhartree(native):
                                                                 f["iterations"]))
         assert f["converged"]
         assert abs(f["energy"] - r["energy"]) < 1e-6
+
+
+def test_batch_sad_start_and_uneven_groups(native):
+    """SAD start, groups of 2 over 5 molecules (the last one alone), two
+    groups in flight and sorting off: every energy within 1e-8 Eh of the
+    core-guess batch and no more iterations; the density of each molecule
+    idempotent in its metric (P S P = P) and of trace n_occ."""
+    from mmd.batch import run_batch_device
+    from mmd.molecule import Molecule
+    geoms = _geoms([3, 1, 4, 2, 5], 1300)
+    ref = run_batch_device(geoms, "6-31g", group=5, conver=1e-10)
+    sad = run_batch_device(geoms, "6-31g", group=2, streams=2, guess="sad", conver=1e-10, keep_density=True,
+                           sort_by_size=False)
+    for g, r, s in zip(geoms, ref, sad):
+        print("core %.10f (%d it)  SAD %.10f (%d it)" % (r["energy"], r["iterations"], s["energy"], s["iterations"]))
+        assert s["converged"]
+        assert abs(s["energy"] - r["energy"]) < 1e-8
+        assert s["iterations"] <= r["iterations"]
+        m = Molecule(g, "6-31g")
+        m.build(direct=True)
+        S = np.real(m.S)
+        P = s["P"]
+        assert abs(np.trace(P @ S) - m.nocc) < 1e-8
+        assert np.max(np.abs(P @ S @ P - P)) < 1e-6
+
+
+def test_batch_input_edges(native):
+    """An empty list gives no results; a molecule above the engine's basis
+    limit (88 functions per molecule) is refused with a clear error."""
+    from mmd.batch import run_batch_device
+    assert run_batch_device([], "6-31g") == []
+    with pytest.raises(Exception, match="basis functions"):
+        run_batch_device([water_cluster(4, 7.0, seed=3)], "cc-pvdz", group=1)

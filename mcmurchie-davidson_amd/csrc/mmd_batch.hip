@@ -57,6 +57,40 @@ __device__ double wg_sum(double v, double* red)
     return s;
 }
 
+typedef double dbl4_t __attribute__((ext_vector_type(4)));
+
+// C (n x n, leading dimension ldc, default n; LDS or global) = op(A) op(B)
+// with A, B n x n in LDS (leading dimension n), on the matrix cores: each wavefront takes 16 x 16 output tiles (the matrix
+// padded with zeros to a multiple of 16) and runs v_mfma_f64_16x16x4_f64
+// over k in steps of 4.  Operand layout (wave64): A[i][k] in lane i + 16k,
+// B[k][j] in lane j + 16k; the accumulator holds D[i][j] in lane j + 16g,
+// element r, with i = g + 4r.
+__device__ void lds_gemm(int n, bool tA, const double* A, bool tB, const double* B, double* C, int ldc = -1)
+{
+    if (ldc < 0) ldc = n;
+    const int nt = (n + 15) >> 4;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nw = (int)(blockDim.x >> 6);
+    const int l16 = lane & 15, l4 = lane >> 4;
+    for (int tile = wave; tile < nt * nt; tile += nw) {
+        const int ti = tile / nt, tj = tile - ti * nt;
+        const int i = ti * 16 + l16, j = tj * 16 + l16;
+        dbl4_t acc = {0.0, 0.0, 0.0, 0.0};
+        for (int k0 = 0; k0 < n; k0 += 4) {
+            const int k = k0 + l4;
+            const double a = (i < n && k < n) ? (tA ? A[k * n + i] : A[i * n + k]) : 0.0;
+            const double b = (k < n && j < n) ? (tB ? B[j * n + k] : B[k * n + j]) : 0.0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        }
+        const int jc = tj * 16 + l16;
+        for (int r = 0; r < 4; ++r) {
+            const int ir = ti * 16 + l4 + 4 * r;
+            if (ir < n && jc < n) C[(size_t)ir * ldc + jc] = acc[r];
+        }
+    }
+    __syncthreads();
+}
+
 // Symmetric eigendecomposition A = V diag(w) V^T of the n x n matrix in LDS
 // (A, leading dimension lda; destroyed) by the cyclic Jacobi method with a
 // round-robin ordering: each round applies n/2 disjoint rotations at once.
@@ -327,17 +361,14 @@ __global__ void __launch_bounds__(BTL) batch_overlap_lds_kernel(BatchArgs B)
     }
     __syncthreads();
     lds_jacobi(n, A, V, L.w, L.perm, L.cs, L.red);
-    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
-        const int i = e / n, j = e - i * n;
-        double x = 0.0, u = 0.0;
-        for (int k = 0; k < n; ++k) {
-            const double vv = V[i * n + k] * V[j * n + k];
-            x = fma(vv, 1.0 / sqrt(L.w[k]), x);
-            u = fma(vv, sqrt(L.w[k]), u);
-        }
-        blk(B.X, N, M.f0)[(size_t)i * N + j] = x;
-        blk(B.U, N, M.f0)[(size_t)i * N + j] = u;
-    }
+    // X = V w^-1/2 V^T and U = V w^1/2 V^T: the scaled eigenvectors in A, then
+    // one product each straight into the molecule's blocks
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) A[e] = V[e] / sqrt(L.w[e % n]);
+    __syncthreads();
+    lds_gemm(n, false, A, true, V, blk(B.X, N, M.f0), N);
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) A[e] = V[e] * sqrt(L.w[e % n]);
+    __syncthreads();
+    lds_gemm(n, false, A, true, V, blk(B.U, N, M.f0), N);
     if (threadIdx.x == 0 && !(L.w[0] > 0.0)) B.flag[blockIdx.x] = -1;   // not positive definite
 }
 
@@ -492,24 +523,6 @@ __global__ void __launch_bounds__(BT) batch_step_kernel(BatchArgs B, int build, 
         else Sx.iterations = step + 1;
     }
     for (int k = threadIdx.x; k < n; k += blockDim.x) B.mo[M.f0 + k] = L.w[k];
-}
-
-// C (LDS, n x n) = op(A) op(B) with A, B n x n in LDS (leading dimension n).
-__device__ void lds_gemm(int n, bool tA, const double* A, bool tB, const double* B, double* C)
-{
-    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
-        const int i = e / n, j = e - i * n;
-        double s = 0.0;
-        if (!tA && !tB) {
-            const double* a = A + i * n;
-            for (int k = 0; k < n; ++k) s = fma(a[k], B[k * n + j], s);
-        } else {
-            for (int k = 0; k < n; ++k)
-                s = fma(tA ? A[k * n + i] : A[i * n + k], tB ? B[j * n + k] : B[k * n + j], s);
-        }
-        C[e] = s;
-    }
-    __syncthreads();
 }
 
 // load the n x n block at M (leading dimension N) into LDS

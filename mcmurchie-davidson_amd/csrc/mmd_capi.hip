@@ -26,6 +26,14 @@
 
 using namespace mmd;
 
+namespace mmd {
+std::recursive_mutex& device_alloc_mutex()
+{
+    static std::recursive_mutex mu;
+    return mu;
+}
+}  // namespace mmd
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -63,13 +71,17 @@ struct DevBuf {
     ~DevBuf() { release(); }
     void release()
     {
-        if (p) (void)hipFree(p);
+        if (p) {
+            std::lock_guard<std::recursive_mutex> lk(device_alloc_mutex());
+            (void)hipFree(p);
+        }
         p = nullptr;
         n = 0;
     }
     hipError_t reserve(size_t count)
     {
         if (count <= n && p) return hipSuccess;
+        std::lock_guard<std::recursive_mutex> lk(device_alloc_mutex());
         release();
         if (count == 0) count = 1;
         hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
@@ -960,10 +972,13 @@ int mmd_ctx_create(mmd_ctx** out, int device_id)
     std::vector<double> tab;
     boys_build_table(tab);
     if (e == hipSuccess) e = c->d_boys.reserve(tab.size());
-    if (e == hipSuccess) e = hipMemcpy(c->d_boys.p, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->d_boys.p, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, c->st);
     std::vector<float> tabf(tab.begin(), tab.end());
     if (e == hipSuccess) e = c->d_boysf.reserve(tabf.size());
-    if (e == hipSuccess) e = hipMemcpy(c->d_boysf.p, tabf.data(), sizeof(float) * tabf.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->d_boysf.p, tabf.data(), sizeof(float) * tabf.size(), hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
     if (e != hipSuccess) {
         mmd_ctx_destroy(c);
         return fail(MMD_ERR_DEVICE, "context setup: %s", hipGetErrorString(e));
@@ -985,7 +1000,7 @@ void mmd_ctx_destroy(mmd_ctx* ctx)
     if (ctx->st) (void)hipStreamSynchronize(ctx->st);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
-    for (int i = 0; i < 2 * NQCLASS; ++i)
+    for (int i = 0; i < 3 * NQCLASS; ++i)
         if (ctx->evc[i]) (void)hipEventDestroy(ctx->evc[i]);
     for (int i = 0; i < mmd_ctx::NSIDE; ++i) {
         if (ctx->side[i]) (void)hipStreamSynchronize(ctx->side[i]);
@@ -993,6 +1008,7 @@ void mmd_ctx_destroy(mmd_ctx* ctx)
         if (ctx->side[i]) (void)hipStreamDestroy(ctx->side[i]);
     }
     if (ctx->evfork) (void)hipEventDestroy(ctx->evfork);
+    std::lock_guard<std::recursive_mutex> lk(device_alloc_mutex());
     if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
     hipStream_t st = ctx->st;
     delete ctx;  // frees device buffers
@@ -1080,7 +1096,8 @@ int mmd_set_atoms(mmd_ctx* ctx, int natom, const double* Z, const double* xyz)
         a[4 * i + 3] = xyz[3 * i + 2];
     }
     HIP_TRY(ctx->d_atoms.reserve(a.size()));
-    HIP_TRY(hipMemcpy(ctx->d_atoms.p, a.data(), sizeof(double) * a.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpyAsync(ctx->d_atoms.p, a.data(), sizeof(double) * a.size(), hipMemcpyHostToDevice, ctx->st));
+    HIP_TRY(hipStreamSynchronize(ctx->st));
     ctx->natom = natom;
     return MMD_OK;
 }
@@ -1907,6 +1924,7 @@ int mmd_batch_rhf(mmd_ctx* ctx, const int32_t* nocc, const double* enuc, const d
     // the iterations are launched one by one
     const bool profiled = c->profiling != 0;
     if (use_graph && !profiled) {
+        std::lock_guard<std::recursive_mutex> lk(device_alloc_mutex());
         if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) == hipSuccess) {
             const int r = iteration();
             hipGraph_t g = nullptr;
@@ -1966,6 +1984,7 @@ int mmd_scf_begin(mmd_ctx* ctx, int nocc, const double* H, const double* S, cons
     if (N == 0) return fail(MMD_ERR_STATE, "no basis set");
     if (nocc < 1 || nocc > N || !H || !S || !X || !F || !P) return fail(MMD_ERR_ARG, "invalid SCF arguments");
     if (!ctx->blas) {
+        std::lock_guard<std::recursive_mutex> lk(device_alloc_mutex());
         if (rocblas_create_handle(&ctx->blas) != rocblas_status_success)
             return fail(MMD_ERR_DEVICE, "rocblas_create_handle failed");
     }

@@ -1,10 +1,17 @@
 // Kernel entry points of libmmd (host-side declarations).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <mutex>
 #include <vector>
 #include "mmd_types.h"
 
 namespace mmd {
+
+// Held around every device allocation and free of the library and around the
+// capture of a HIP graph: allocation calls of one thread must not interleave
+// with a stream capture running in another (contexts run in parallel threads).
+std::recursive_mutex& device_alloc_mutex();
+
 
 // Quartet class index: qc = pcb (pcb+1)/2 + pck with pcb >= pck.
 inline __host__ __device__ void qclass_decode(int qc, int* la, int* lb, int* lc, int* ld)

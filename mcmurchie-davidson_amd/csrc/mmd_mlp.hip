@@ -10,6 +10,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <algorithm>
+#include <mutex>
 #include <new>
 #include <random>
 #include <string>
@@ -18,6 +19,7 @@
 
 namespace mmd {
 void set_last_error(const char* msg);   // the library's mmd_last_error() text
+std::recursive_mutex& device_alloc_mutex();   // held around allocations and graph captures
 }
 
 namespace {
@@ -38,11 +40,15 @@ struct Buf {
     }
     ~Buf()
     {
-        if (p) (void)hipFree(p);
+        if (p) {
+            std::lock_guard<std::recursive_mutex> lk(mmd::device_alloc_mutex());
+            (void)hipFree(p);
+        }
     }
     hipError_t reserve(size_t count)
     {
         if (count <= n && p) return hipSuccess;
+        std::lock_guard<std::recursive_mutex> lk(mmd::device_alloc_mutex());
         if (p) (void)hipFree(p);
         p = nullptr;
         n = 0;

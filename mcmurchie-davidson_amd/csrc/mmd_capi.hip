@@ -165,6 +165,11 @@ struct mmd_ctx {
     std::vector<double> sh_exp, sh_w, gcomp;
     DevBuf<double> d_gcomp;
     DevBuf<int> d_sh_bf0, d_sh_nbf;
+    DevBuf<OneeShell> d_oshell;         // shells for the one-electron kernels
+    DevBuf<double> d_sh_exp, d_sh_w;
+    DevBuf<int2> d_onee_list;           // every shell pair of each molecule, by pair class
+    int onee_off[NPAIRCLASS + 1] = {0};
+    bool onee_list_ok = false;
 
     // shell pairs (grouped by pair class, sorted inside each class)
     bool pairs_ok = false;
@@ -347,7 +352,21 @@ int upload_shells(mmd_ctx* c)
     HIP_TRY(c->d_sh_nbf.reserve(ns));
     HIP_TRY(hipMemcpyAsync(c->d_sh_bf0.p, b0.data(), sizeof(int) * ns, hipMemcpyHostToDevice, c->st));
     HIP_TRY(hipMemcpyAsync(c->d_sh_nbf.p, nb.data(), sizeof(int) * ns, hipMemcpyHostToDevice, c->st));
+    std::vector<OneeShell> os(std::max(ns, 1));
+    for (int k = 0; k < ns; ++k) {
+        const Shell& sh = c->shells[k];
+        os[k] = OneeShell{sh.x, sh.y, sh.z, sh.l, sh.np, sh.p0, sh.bf0, sh.mol, 0};
+    }
+    HIP_TRY(c->d_oshell.reserve(os.size()));
+    HIP_TRY(c->d_sh_exp.reserve(c->sh_exp.size()));
+    HIP_TRY(c->d_sh_w.reserve(c->sh_w.size()));
+    HIP_TRY(hipMemcpyAsync(c->d_oshell.p, os.data(), sizeof(OneeShell) * os.size(), hipMemcpyHostToDevice, c->st));
+    HIP_TRY(hipMemcpyAsync(c->d_sh_exp.p, c->sh_exp.data(), sizeof(double) * c->sh_exp.size(), hipMemcpyHostToDevice,
+                           c->st));
+    HIP_TRY(hipMemcpyAsync(c->d_sh_w.p, c->sh_w.data(), sizeof(double) * c->sh_w.size(), hipMemcpyHostToDevice,
+                           c->st));
     HIP_TRY(hipStreamSynchronize(c->st));
+    c->onee_list_ok = false;
     return MMD_OK;
 }
 
@@ -382,6 +401,62 @@ int molecule_shell_ranges(mmd_ctx* c)
         HIP_TRY(c->d_mol_dmax.reserve(c->nmol));
         HIP_TRY(hipMemcpyAsync(c->d_mol_blocks.p, blk.data(), sizeof(int4) * c->nmol, hipMemcpyHostToDevice, c->st));
     }
+    return MMD_OK;
+}
+
+// One-electron integrals of the basis on the shell-pair kernels: S, T, V
+// (N x N) and, when M and Lm are given, the dipole and angular-momentum
+// integrals about `center`.  A basis of several molecules gets only the
+// molecules' own blocks; the rest of the outputs is cleared.
+int one_electron_shells(mmd_ctx* c, const double center[3], double* S, double* T, double* V, double* M, double* Lm)
+{
+    if (!c->onee_list_ok) {
+        if (int rc = molecule_shell_ranges(c)) return rc;
+        std::vector<std::vector<int2>> byclass(NPAIRCLASS);
+        for (int m = 0; m < c->nmol; ++m)
+            for (int I = c->mol_s0[m]; I < c->mol_s0[m + 1]; ++I)
+                for (int J = c->mol_s0[m]; J <= I; ++J) {
+                    int a = I, b = J;
+                    if (c->shells[I].l < c->shells[J].l) std::swap(a, b);
+                    byclass[pair_class(c->shells[a].l, c->shells[b].l)].push_back(make_int2(a, b));
+                }
+        std::vector<int2> all;
+        for (int pc = 0; pc < NPAIRCLASS; ++pc) {
+            c->onee_off[pc] = (int)all.size();
+            all.insert(all.end(), byclass[pc].begin(), byclass[pc].end());
+        }
+        c->onee_off[NPAIRCLASS] = (int)all.size();
+        HIP_TRY(c->d_onee_list.reserve(std::max<size_t>(all.size(), 1)));
+        HIP_TRY(hipMemcpyAsync(c->d_onee_list.p, all.data(), sizeof(int2) * all.size(), hipMemcpyHostToDevice,
+                               c->st));
+        HIP_TRY(hipStreamSynchronize(c->st));
+        c->onee_list_ok = true;
+    }
+    const size_t nn = (size_t)c->nbf * c->nbf;
+    if (c->nmol > 1) {
+        for (double* x : {S, T, V}) HIP_TRY(launch_fill(nn, x, 0.0, c->st));
+        if (M && Lm)
+            for (double* x : {M, Lm}) HIP_TRY(launch_fill(3 * nn, x, 0.0, c->st));
+    }
+    OneeArgs A;
+    A.shells = c->d_oshell.p;
+    A.exps = c->d_sh_exp.p;
+    A.wts = c->d_sh_w.p;
+    A.gcomp = c->d_gcomp.p;
+    A.atoms = c->natom ? c->d_atoms.p : nullptr;
+    A.natom = c->natom;
+    A.mol_atoms = c->nmol > 1 ? c->d_mol_atoms.p : nullptr;
+    A.boys = c->d_boys.p;
+    A.cx = center[0];
+    A.cy = center[1];
+    A.cz = center[2];
+    A.nbf = c->nbf;
+    A.S = S;
+    A.T = T;
+    A.V = V;
+    A.M = M;
+    A.Lm = Lm;
+    HIP_TRY(launch_one_electron_shells(A, c->d_onee_list.p, c->onee_off, c->st));
     return MMD_OK;
 }
 
@@ -1205,10 +1280,16 @@ int mmd_one_electron(mmd_ctx* ctx, const double center[3], double* S, double* T,
     DevBuf<double>& d = ctx->d_onee;   // kept between calls (no allocation per SCF)
     HIP_TRY(d.reserve(9 * nn));
     const bool several = ctx->nmol > 1;
-    HIP_TRY(launch_one_electron(ctx->nbf, ctx->d_bf.p, ctx->d_bf_exp.p, ctx->d_bf_w.p, ctx->natom,
-                                ctx->natom ? ctx->d_atoms.p : nullptr, center[0], center[1], center[2], ctx->d_boys.p,
-                                d.p, d.p + nn, d.p + 2 * nn, d.p + 3 * nn, d.p + 6 * nn, ctx->st,
-                                several ? ctx->d_plist.p : nullptr, ctx->npl, several ? ctx->d_mol_atoms.p : nullptr));
+    if (ctx->shells_ok) {
+        if ((rc = one_electron_shells(ctx, center, d.p, d.p + nn, d.p + 2 * nn, d.p + 3 * nn, d.p + 6 * nn)))
+            return rc;
+    } else {
+        HIP_TRY(launch_one_electron(ctx->nbf, ctx->d_bf.p, ctx->d_bf_exp.p, ctx->d_bf_w.p, ctx->natom,
+                                    ctx->natom ? ctx->d_atoms.p : nullptr, center[0], center[1], center[2],
+                                    ctx->d_boys.p, d.p, d.p + nn, d.p + 2 * nn, d.p + 3 * nn, d.p + 6 * nn, ctx->st,
+                                    several ? ctx->d_plist.p : nullptr, ctx->npl,
+                                    several ? ctx->d_mol_atoms.p : nullptr));
+    }
     HIP_TRY(hipMemcpyAsync(S, d.p, sizeof(double) * nn, hipMemcpyDeviceToHost, ctx->st));
     HIP_TRY(hipMemcpyAsync(T, d.p + nn, sizeof(double) * nn, hipMemcpyDeviceToHost, ctx->st));
     HIP_TRY(hipMemcpyAsync(V, d.p + 2 * nn, sizeof(double) * nn, hipMemcpyDeviceToHost, ctx->st));
@@ -1807,12 +1888,12 @@ int mmd_batch_rhf(mmd_ctx* ctx, const int32_t* nocc, const double* enuc, const d
     }
     hipStream_t st = c->st;
     // one-electron integrals of every molecule's own block; H = T + V
-    HIP_TRY(c->d_onee.reserve(9 * nn));
+    HIP_TRY(c->d_onee.reserve(3 * nn));
     double* S = c->d_onee.p;
-    HIP_TRY(launch_one_electron(N, c->d_bf.p, c->d_bf_exp.p, c->d_bf_w.p, c->natom,
-                                c->natom ? c->d_atoms.p : nullptr, 0.0, 0.0, 0.0, c->d_boys.p, S, S + nn, S + 2 * nn,
-                                S + 3 * nn, S + 6 * nn, st, M > 1 ? c->d_plist.p : nullptr, c->npl,
-                                M > 1 ? c->d_mol_atoms.p : nullptr));
+    {
+        const double origin[3] = {0.0, 0.0, 0.0};
+        if ((rc = one_electron_shells(c, origin, S, S + nn, S + 2 * nn, nullptr, nullptr))) return rc;
+    }
     for (DevBuf<double>* b : {&c->b_H, &c->b_X, &c->b_U, &c->b_F, &c->b_P, &c->b_Pold, &c->b_dP, &c->b_G})
         HIP_TRY(b->reserve(nn));
     HIP_TRY(c->b_diis.reserve(std::max<size_t>(1, dtot)));

@@ -87,6 +87,31 @@ hipError_t launch_one_electron(int nbf, const BFDesc* bfs, const double* pexp, c
                                const double* boys, double* S, double* T, double* V, double* M, double* Lm,
                                hipStream_t st, const int2* plist = nullptr, int64_t npl = 0,
                                const int2* mol_atoms = nullptr /* per molecule: first atom, count */);
+// One-electron integrals by shell pair (compile-time angular momenta per
+// pair class): a shell with its primitives at exps/wts[p0 .. p0+np) (the
+// weights of its first Cartesian component; gcomp scales the others).
+struct OneeShell {
+    double x, y, z;
+    int32_t l, np, p0, bf0, mol, pad;
+};
+struct OneeArgs {
+    const OneeShell* shells;
+    const double* exps;
+    const double* wts;
+    const double* gcomp;
+    const double* atoms;       // natom x 4: Z, x, y, z
+    int natom;
+    const int2* mol_atoms;     // per molecule: first atom, count; or null (every atom)
+    const double* boys;
+    double cx, cy, cz;         // origin of the multipoles
+    int nbf;
+    double *S, *T, *V;         // N x N
+    double *M, *Lm;            // 3 x N x N each, or null (not computed)
+};
+// list: shell pairs (a, b), l_a >= l_b, of pair class pc at
+// list[off[pc] .. off[pc+1]) (off on the host).  Writes the pairs' blocks
+// and their transposes; the other elements are left as they are.
+hipError_t launch_one_electron_shells(const OneeArgs& A, const int2* list, const int* off, hipStream_t st);
 // Survivor list of one class; with surv2, the quartets whose bound lies in
 // [tol, split) go to the second list instead (the FP32 tier).
 hipError_t launch_screen_class(int boff, int nb, int koff, int nk, int tri, const ShellPair* pairs,

@@ -44,16 +44,20 @@ FP64_PEAK_TFLOPS = 78.6
 # counters read for a copy of known size (recorded with the command lines in
 # profiles/); a class absent here reports null.
 TRAFFIC = {
-    (0, 0, 0, 0): 2.242e+07, (1, 0, 0, 0): 4.229e+07, (1, 0, 1, 0): 4.499e+07, (1, 1, 0, 0): 2.063e+07,
-    (1, 1, 1, 0): 5.513e+07, (1, 1, 1, 1): 7.862e+06, (2, 0, 0, 0): 1.916e+07, (2, 0, 1, 0): 3.828e+07,
-    (2, 0, 1, 1): 1.154e+07, (2, 0, 2, 0): 5.946e+06, (2, 1, 0, 0): 1.251e+07, (2, 1, 1, 0): 1.652e+07,
-    (2, 1, 1, 1): 7.645e+06, (2, 1, 2, 0): 5.551e+06, (2, 1, 2, 1): 6.124e+06, (2, 2, 0, 0): 3.453e+06,
-    (2, 2, 1, 0): 5.270e+06, (2, 2, 1, 1): 2.754e+06, (2, 2, 2, 0): 2.329e+06, (2, 2, 2, 1): 5.559e+06,
+    (0, 0, 0, 0): 2.239e+07, (1, 0, 0, 0): 4.238e+07, (1, 0, 1, 0): 4.503e+07, (1, 1, 0, 0): 2.112e+07,
+    (1, 1, 1, 0): 5.566e+07, (1, 1, 1, 1): 7.852e+06, (2, 0, 0, 0): 1.826e+07, (2, 0, 1, 0): 3.815e+07,
+    (2, 0, 1, 1): 1.128e+07, (2, 0, 2, 0): 5.933e+06, (2, 1, 0, 0): 1.416e+07, (2, 1, 1, 0): 1.686e+07,
+    (2, 1, 1, 1): 3.417e+07, (2, 1, 2, 0): 3.096e+07, (2, 1, 2, 1): 7.055e+06, (2, 2, 0, 0): 3.465e+06,
+    (2, 2, 1, 0): 4.796e+06, (2, 2, 1, 1): 6.789e+06, (2, 2, 2, 0): 5.410e+06, (2, 2, 2, 1): 3.559e+06,
+    (2, 2, 2, 2): 1.207e+06,
 }
 
 
 # The same for the dominant class kernel of the headline workload.
-TRAFFIC_HEADLINE = {}
+TRAFFIC_HEADLINE = {
+    (0, 0, 0, 0): 3.116e+08, (1, 0, 0, 0): 3.929e+08, (1, 0, 1, 0): 1.411e+08, (1, 1, 0, 0): 1.788e+08,
+    (1, 1, 1, 0): 1.162e+08, (1, 1, 1, 1): 1.994e+07,
+}
 
 
 def ncart(l):

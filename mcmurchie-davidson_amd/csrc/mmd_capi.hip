@@ -404,41 +404,36 @@ int molecule_shell_ranges(mmd_ctx* c)
     return MMD_OK;
 }
 
-// One-electron integrals of the basis on the shell-pair kernels: S, T, V
-// (N x N) and, when M and Lm are given, the dipole and angular-momentum
-// integrals about `center`.  A basis of several molecules gets only the
-// molecules' own blocks; the rest of the outputs is cleared.
-int one_electron_shells(mmd_ctx* c, const double center[3], double* S, double* T, double* V, double* M, double* Lm)
+// Every shell pair of each molecule (l_a >= l_b), grouped by pair class,
+// for the shell-pair one-electron and Schwarz kernels; built once per basis.
+int onee_pair_list(mmd_ctx* c)
 {
-    if (!c->onee_list_ok) {
-        if (int rc = molecule_shell_ranges(c)) return rc;
-        std::vector<std::vector<int2>> byclass(NPAIRCLASS);
-        for (int m = 0; m < c->nmol; ++m)
-            for (int I = c->mol_s0[m]; I < c->mol_s0[m + 1]; ++I)
-                for (int J = c->mol_s0[m]; J <= I; ++J) {
-                    int a = I, b = J;
-                    if (c->shells[I].l < c->shells[J].l) std::swap(a, b);
-                    byclass[pair_class(c->shells[a].l, c->shells[b].l)].push_back(make_int2(a, b));
-                }
-        std::vector<int2> all;
-        for (int pc = 0; pc < NPAIRCLASS; ++pc) {
-            c->onee_off[pc] = (int)all.size();
-            all.insert(all.end(), byclass[pc].begin(), byclass[pc].end());
-        }
-        c->onee_off[NPAIRCLASS] = (int)all.size();
-        HIP_TRY(c->d_onee_list.reserve(std::max<size_t>(all.size(), 1)));
-        HIP_TRY(hipMemcpyAsync(c->d_onee_list.p, all.data(), sizeof(int2) * all.size(), hipMemcpyHostToDevice,
-                               c->st));
-        HIP_TRY(hipStreamSynchronize(c->st));
-        c->onee_list_ok = true;
+    if (c->onee_list_ok) return MMD_OK;
+    if (int rc = molecule_shell_ranges(c)) return rc;
+    std::vector<std::vector<int2>> byclass(NPAIRCLASS);
+    for (int m = 0; m < c->nmol; ++m)
+        for (int I = c->mol_s0[m]; I < c->mol_s0[m + 1]; ++I)
+            for (int J = c->mol_s0[m]; J <= I; ++J) {
+                int a = I, b = J;
+                if (c->shells[I].l < c->shells[J].l) std::swap(a, b);
+                byclass[pair_class(c->shells[a].l, c->shells[b].l)].push_back(make_int2(a, b));
+            }
+    std::vector<int2> all;
+    for (int pc = 0; pc < NPAIRCLASS; ++pc) {
+        c->onee_off[pc] = (int)all.size();
+        all.insert(all.end(), byclass[pc].begin(), byclass[pc].end());
     }
-    const size_t nn = (size_t)c->nbf * c->nbf;
-    if (c->nmol > 1) {
-        for (double* x : {S, T, V}) HIP_TRY(launch_fill(nn, x, 0.0, c->st));
-        if (M && Lm)
-            for (double* x : {M, Lm}) HIP_TRY(launch_fill(3 * nn, x, 0.0, c->st));
-    }
-    OneeArgs A;
+    c->onee_off[NPAIRCLASS] = (int)all.size();
+    HIP_TRY(c->d_onee_list.reserve(std::max<size_t>(all.size(), 1)));
+    HIP_TRY(hipMemcpyAsync(c->d_onee_list.p, all.data(), sizeof(int2) * all.size(), hipMemcpyHostToDevice, c->st));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    c->onee_list_ok = true;
+    return MMD_OK;
+}
+
+OneeArgs onee_args(const mmd_ctx* c)
+{
+    OneeArgs A{};
     A.shells = c->d_oshell.p;
     A.exps = c->d_sh_exp.p;
     A.wts = c->d_sh_w.p;
@@ -447,16 +442,50 @@ int one_electron_shells(mmd_ctx* c, const double center[3], double* S, double* T
     A.natom = c->natom;
     A.mol_atoms = c->nmol > 1 ? c->d_mol_atoms.p : nullptr;
     A.boys = c->d_boys.p;
+    A.nbf = c->nbf;
+    return A;
+}
+
+// One-electron integrals of the basis on the shell-pair kernels: S, T, V
+// (N x N) and, when M and Lm are given, the dipole and angular-momentum
+// integrals about `center`.  A basis of several molecules gets only the
+// molecules' own blocks; the rest of the outputs is cleared.
+int one_electron_shells(mmd_ctx* c, const double center[3], double* S, double* T, double* V, double* M, double* Lm)
+{
+    if (int rc = onee_pair_list(c)) return rc;
+    const size_t nn = (size_t)c->nbf * c->nbf;
+    if (c->nmol > 1) {
+        for (double* x : {S, T, V}) HIP_TRY(launch_fill(nn, x, 0.0, c->st));
+        if (M && Lm)
+            for (double* x : {M, Lm}) HIP_TRY(launch_fill(3 * nn, x, 0.0, c->st));
+    }
+    OneeArgs A = onee_args(c);
     A.cx = center[0];
     A.cy = center[1];
     A.cz = center[2];
-    A.nbf = c->nbf;
     A.S = S;
     A.T = T;
     A.V = V;
     A.M = M;
     A.Lm = Lm;
     HIP_TRY(launch_one_electron_shells(A, c->d_onee_list.p, c->onee_off, c->st));
+    return MMD_OK;
+}
+
+// The Schwarz diagonal (pq|pq) of the basis into out (N x N; for several
+// molecules, zero outside their blocks): shell-pair kernels where the basis
+// forms complete shells, else the per-function kernel.
+int schwarz_diagonal(mmd_ctx* c, double* out)
+{
+    const int N = c->nbf;
+    if (!c->shells_ok) {
+        HIP_TRY(launch_eri_diag(N, c->d_bf.p, c->d_bf_exp.p, c->d_bf_w.p, c->d_boys.p, out, c->st,
+                                c->nmol > 1 ? c->d_plist.p : nullptr, c->npl));
+        return MMD_OK;
+    }
+    if (int rc = onee_pair_list(c)) return rc;
+    if (c->nmol > 1) HIP_TRY(launch_fill((size_t)N * N, out, 0.0, c->st));
+    HIP_TRY(launch_schwarz_shells(onee_args(c), c->d_onee_list.p, c->onee_off, out, c->st));
     return MMD_OK;
 }
 
@@ -1308,8 +1337,7 @@ int mmd_schwarz(mmd_ctx* ctx, double* diag)
     const size_t nn = (size_t)N * N;
     std::vector<double> h(nn);
     HIP_TRY(ctx->d_schwarz.reserve(nn));
-    HIP_TRY(launch_eri_diag(N, ctx->d_bf.p, ctx->d_bf_exp.p, ctx->d_bf_w.p, ctx->d_boys.p, ctx->d_schwarz.p, ctx->st,
-                            ctx->nmol > 1 ? ctx->d_plist.p : nullptr, ctx->npl));
+    if ((rc = schwarz_diagonal(ctx, ctx->d_schwarz.p))) return rc;
     HIP_TRY(hipMemcpyAsync(h.data(), ctx->d_schwarz.p, sizeof(double) * nn, hipMemcpyDeviceToHost, ctx->st));
     HIP_TRY(hipStreamSynchronize(ctx->st));
     if (diag) memcpy(diag, h.data(), sizeof(double) * nn);
@@ -1906,8 +1934,7 @@ int mmd_batch_rhf(mmd_ctx* ctx, const int32_t* nocc, const double* enuc, const d
     HIP_TRY(hipGetLastError());
     // Schwarz diagonal, pair tables and work units of the whole batch
     HIP_TRY(c->d_schwarz.reserve(nn));
-    HIP_TRY(launch_eri_diag(N, c->d_bf.p, c->d_bf_exp.p, c->d_bf_w.p, c->d_boys.p, c->d_schwarz.p, st,
-                            M > 1 ? c->d_plist.p : nullptr, c->npl));
+    if ((rc = schwarz_diagonal(c, c->d_schwarz.p))) return rc;
     HIP_TRY(c->b_mol.reserve(M));
     HIP_TRY(hipMemcpyAsync(c->b_mol.p, mols.data(), sizeof(BatchMol) * M, hipMemcpyHostToDevice, st));
     BatchArgs B;

@@ -112,6 +112,9 @@ struct OneeArgs {
 // list[off[pc] .. off[pc+1]) (off on the host).  Writes the pairs' blocks
 // and their transposes; the other elements are left as they are.
 hipError_t launch_one_electron_shells(const OneeArgs& A, const int2* list, const int* off, hipStream_t st);
+// The Schwarz diagonal out[i][j] = (ij|ij) of the same shell pairs (only
+// A.shells, exps, wts, gcomp, boys and nbf are read).
+hipError_t launch_schwarz_shells(const OneeArgs& A, const int2* list, const int* off, double* out, hipStream_t st);
 // Survivor list of one class; with surv2, the quartets whose bound lies in
 // [tol, split) go to the second list instead (the FP32 tier).
 hipError_t launch_screen_class(int boff, int nb, int koff, int nk, int tri, const ShellPair* pairs,

@@ -13,8 +13,14 @@
 //   E^{i,j+2}_0 = (X_PB^2 + 1/(2p)) E_0 + 2 X_PB E_1 + 2 E_2,
 // with E_t = E^{ij}_t.
 //
+// The Schwarz diagonal (ab|ab) of every component pair of a shell pair is
+// computed the same way (schwarz_pair_kernel), with lane items (bra
+// primitive pair, ket primitive pair) of the pair with itself.
+//
 // Reference semantics: cython/onee.pyx:14-192 (the primitive integrals and
-// their contraction), mmd/molecule.py:253-276 (assembly of S, T, V, M, L).
+// their contraction), mmd/molecule.py:253-276 (assembly of S, T, V, M, L),
+// mmd/molecule.py:93-99 and cython/twoe.pyx:34-94 (the diagonal ERIs of the
+// Schwarz screen).
 #include <hip/hip_runtime.h>
 
 #include "mmd_device.h"
@@ -177,6 +183,111 @@ __global__ void __launch_bounds__(256) onee_pair_kernel(OneeArgs A, const int2* 
     });
 }
 
+
+// (ab|ab) for every component pair of the shell pair list[w] (one
+// wavefront); out[i][j] = out[j][i] = (ij|ij).
+template <int LA, int LB>
+__global__ void __launch_bounds__(256) schwarz_pair_kernel(OneeArgs A, const int2* __restrict__ list, int npairs,
+                                                           double* __restrict__ out)
+{
+    constexpr int NA = ncart(LA), NB = ncart(LB), NAB = NA * NB, LAB = LA + LB, L = 2 * LAB;
+    constexpr int NT = LAB + 1;
+    constexpr int NE = (LA + 1) * (LB + 1) * NT;
+    constexpr double TWO_PI_2_5 = 34.98683665524972497;   // 2 pi^{5/2}
+    const int wid = (int)(((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
+    if (wid >= npairs) return;   // uniform across the wavefront
+    const int2 pr = list[wid];
+    const OneeShell a = A.shells[pr.x], b = A.shells[pr.y];
+    const double Ac[3] = {a.x, a.y, a.z}, Bc[3] = {b.x, b.y, b.z};
+    const double AB2 = (a.x - b.x) * (a.x - b.x) + (a.y - b.y) * (a.y - b.y) + (a.z - b.z) * (a.z - b.z);
+    const int npp = a.np * b.np;
+    double acc[NAB];
+    sfor<NAB>([&](auto k) { acc[decltype(k)::value] = 0.0; });
+    // primitive pair k of the shell pair: exponent sum, weight x K_AB, centre
+    auto prim = [&](int k, double& p, double& w, double* P) {
+        const int ia = k / b.np, ib = k - (k / b.np) * b.np;
+        const double ea = A.exps[a.p0 + ia], eb = A.exps[b.p0 + ib];
+        p = ea + eb;
+        const double ip = 1.0 / p;
+        w = A.wts[a.p0 + ia] * A.wts[b.p0 + ib] * exp(-ea * eb * ip * AB2);
+        P[0] = (ea * a.x + eb * b.x) * ip;
+        P[1] = (ea * a.y + eb * b.y) * ip;
+        P[2] = (ea * a.z + eb * b.z) * ip;
+    };
+    for (int it = lane; it < npp * npp; it += 64) {
+        const int i = it / npp, j = it - (it / npp) * npp;
+        double p, wp, P[3], q, wq, Q[3];
+        prim(i, p, wp, P);
+        prim(j, q, wq, Q);
+        double Ep[3][NE], Eq[3][NE];
+        sfor<3>([&](auto xx) {
+            constexpr int x = decltype(xx)::value;
+            hermite_E<LA, LB, 1, double>(0.5 / p, P[x] - Ac[x], P[x] - Bc[x], Ep[x]);
+            hermite_E<LA, LB, -1, double>(0.5 / q, Q[x] - Ac[x], Q[x] - Bc[x], Eq[x]);
+        });
+        const double rs = rsqrt(p + q);
+        const double alpha = p * q * rs * rs;
+        const double X = P[0] - Q[0], Y = P[1] - Q[1], Z = P[2] - Q[2];
+        double F[L + 1];
+        boys_seeds<L, double>(alpha * (X * X + Y * Y + Z * Z), alpha, TWO_PI_2_5 * wp * wq / (p * q) * rs,
+                              A.boys + (size_t)L * BOYS_NT, F);
+        double R[nherm(L)];
+        hermite_R<L, double>(X, Y, Z, F, R);
+        sfor<NAB>([&](auto kk) {
+            constexpr int k = decltype(kk)::value;
+            constexpr int ca = k / NB, cb = k % NB;
+            constexpr int ex = (comp_lx(LA, ca) * (LB + 1) + comp_lx(LB, cb)) * NT;
+            constexpr int ey = (comp_ly(LA, ca) * (LB + 1) + comp_ly(LB, cb)) * NT;
+            constexpr int ez = (comp_lz(LA, ca) * (LB + 1) + comp_lz(LB, cb)) * NT;
+            constexpr int nx = comp_lx(LA, ca) + comp_lx(LB, cb) + 1;
+            constexpr int ny = comp_ly(LA, ca) + comp_ly(LB, cb) + 1;
+            constexpr int nz = comp_lz(LA, ca) + comp_lz(LB, cb) + 1;
+            double val = 0.0;
+            sfor<nx>([&](auto t_) {
+                constexpr int t = decltype(t_)::value;
+                sfor<ny>([&](auto u_) {
+                    constexpr int u = decltype(u_)::value;
+                    sfor<nz>([&](auto v_) {
+                        constexpr int v = decltype(v_)::value;
+                        // sum over the ket's Hermite indices for bra index (t,u,v)
+                        double s = 0.0;
+                        sfor<nx>([&](auto ta_) {
+                            constexpr int ta = decltype(ta_)::value;
+                            sfor<ny>([&](auto ua_) {
+                                constexpr int ua = decltype(ua_)::value;
+                                const double exy = Eq[0][ex + ta] * Eq[1][ey + ua];
+                                sfor<nz>([&](auto va_) {
+                                    constexpr int va = decltype(va_)::value;
+                                    s = fma(exy * Eq[2][ez + va], R[hidx(t + ta, u + ua, v + va)], s);
+                                });
+                            });
+                        });
+                        val = fma(Ep[0][ex + t] * Ep[1][ey + u] * Ep[2][ez + v], s, val);
+                    });
+                });
+            });
+            acc[k] += val;
+        });
+    }
+    sfor<NAB>([&](auto k) {
+        double v = acc[decltype(k)::value];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        acc[decltype(k)::value] = v;
+    });
+    if (lane != 0) return;
+    const size_t n = (size_t)A.nbf;
+    const bool same = pr.x == pr.y;
+    sfor<NAB>([&](auto kk) {
+        constexpr int k = decltype(kk)::value;
+        constexpr int ca = k / NB, cb = k % NB;
+        if (same && ca < cb) return;
+        const size_t i = (size_t)a.bf0 + ca, j = (size_t)b.bf0 + cb;
+        const double g = A.gcomp[i] * A.gcomp[j];
+        out[i * n + j] = out[j * n + i] = g * g * acc[k];
+    });
+}
+
 template <int LA, int LB, bool MULT>
 hipError_t launch_pc(const OneeArgs& A, const int2* list, int n, hipStream_t st)
 {
@@ -186,7 +297,31 @@ hipError_t launch_pc(const OneeArgs& A, const int2* list, int n, hipStream_t st)
     return hipGetLastError();
 }
 
+template <int LA, int LB>
+hipError_t launch_schwarz_pc(const OneeArgs& A, const int2* list, int n, double* out, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL((schwarz_pair_kernel<LA, LB>), dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, A, list, n,
+                       out);
+    return hipGetLastError();
+}
+
 }  // namespace
+
+hipError_t launch_schwarz_shells(const OneeArgs& A, const int2* list, const int* off, double* out, hipStream_t st)
+{
+    hipError_t e = hipSuccess;
+#define MMD_SCHWARZ_PC(pc, la, lb) \
+    if (e == hipSuccess) e = launch_schwarz_pc<la, lb>(A, list + off[pc], off[pc + 1] - off[pc], out, st);
+    MMD_SCHWARZ_PC(0, 0, 0)
+    MMD_SCHWARZ_PC(1, 1, 0)
+    MMD_SCHWARZ_PC(2, 1, 1)
+    MMD_SCHWARZ_PC(3, 2, 0)
+    MMD_SCHWARZ_PC(4, 2, 1)
+    MMD_SCHWARZ_PC(5, 2, 2)
+#undef MMD_SCHWARZ_PC
+    return e;
+}
 
 hipError_t launch_one_electron_shells(const OneeArgs& A, const int2* list, const int* off, hipStream_t st)
 {

@@ -244,6 +244,7 @@ struct mmd_ctx {
     // class qc owns units [unit_off[qc], unit_off[qc+1]) with ket segment
     // length unit_seg[qc].
     DevBuf<int4> d_units;
+    DevBuf<UnitBound> d_ubound;         // per unit, in the same order
     int64_t unit_off[NQCLASS + 1] = {0};
     int unit_seg[NQCLASS] = {0};
     bool units_ok = false;
@@ -648,8 +649,9 @@ void class_of(int qc, int* pcb, int* pck);
 int build_units(mmd_ctx* c)
 {
     std::vector<int4> all;
+    std::vector<UnitBound> allb;
     std::vector<std::pair<double, int4>> v;
-    std::vector<double> kpre;
+    std::vector<double> kpre, kqmax;
     for (int qc = 0; qc < NQCLASS; ++qc) {
         c->unit_off[qc] = (int64_t)all.size();
         int pcb, pck;
@@ -668,6 +670,13 @@ int build_units(mmd_ctx* c)
         if (nb == 0 || nk == 0) continue;
         kpre.assign(nk + 1, 0.0);
         for (int k = 0; k < nk; ++k) kpre[k + 1] = kpre[k] + c->pairs[koff + k].npp;
+        // largest ket pair factor of a segment [k0, k1): kets are sorted by
+        // molecule, so a running maximum restarted at each segment start
+        auto seg_qmax = [&](int k0, int k1) {
+            double m = 0.0;
+            for (int k = k0; k < k1; ++k) m = std::max(m, c->pairs[koff + k].q);
+            return m;
+        };
         v.clear();
         for (int P = 0; P < nb; ++P) {
             const ShellPair& bp = c->pairs[boff + P];
@@ -688,12 +697,23 @@ int build_units(mmd_ctx* c)
         std::stable_sort(v.begin(), v.end(), [](const std::pair<double, int4>& x, const std::pair<double, int4>& y) {
             return x.first > y.first;
         });
-        for (auto& e : v) all.push_back(e.second);
+        for (auto& e : v) {
+            all.push_back(e.second);
+            const ShellPair& bp = c->pairs[boff + e.second.x];
+            const double qb = bp.q * seg_qmax(e.second.z, e.second.w);
+            float qf = (float)qb;
+            if ((double)qf < qb) qf = nextafterf(qf, HUGE_VALF);
+            allb.push_back(UnitBound{qf, bp.mol});
+        }
     }
     c->unit_off[NQCLASS] = (int64_t)all.size();
     HIP_TRY(c->d_units.reserve(std::max<size_t>(1, all.size())));
-    if (!all.empty())
+    HIP_TRY(c->d_ubound.reserve(std::max<size_t>(1, allb.size())));
+    if (!all.empty()) {
         HIP_TRY(hipMemcpyAsync(c->d_units.p, all.data(), sizeof(int4) * all.size(), hipMemcpyHostToDevice, c->st));
+        HIP_TRY(hipMemcpyAsync(c->d_ubound.p, allb.data(), sizeof(UnitBound) * allb.size(), hipMemcpyHostToDevice,
+                               c->st));
+    }
     HIP_TRY(hipStreamSynchronize(c->st));
     c->units_ok = true;
     return MMD_OK;
@@ -881,6 +901,7 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol, FockScratch& W0)
             R.seg = c->unit_seg[qc];
             R.rw = c->rw;
             R.units = c->d_units.p + c->unit_off[qc];
+            R.ubound = c->d_ubound.p + c->unit_off[qc];
             const int64_t units = c->unit_off[qc + 1] - c->unit_off[qc];
             R.nunits = (int)units;
             if (units == 0) {

@@ -474,15 +474,19 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
         }
         // unit: bra pair P, its primitive pairs [ip0, ip1) and the ket
         // pairs [k0, k1) of its molecule
+        // a unit of a batch whose every quartet fails the screen: pair
+        // factors q_P q_Q <= the unit's bound, and the density weight of the
+        // screen is at most 4 max |dP| of the molecule (0 once converged)
+        if (R.mol_dmax) {
+            const UnitBound ub = R.ubound[u];
+            if ((double)ub.q * 4.0 * R.mol_dmax[ub.mol] < 0.999999 * A.tol) continue;
+        }
         const int4 U = R.units[u];
         const int P = U.x, k0 = U.z;
         const int ip0u = U.y & 0xffff, ip1u = U.y >> 16;
         const int segn = U.w - U.z;
         const int bnp = ip1u - ip0u;   // bra primitive pairs of the unit (<= PSLICE)
         const ShellPair bra = A.pairs[R.boff + P];
-        // a molecule of a batch whose density no longer changes (converged)
-        // contributes nothing: its quartets would all fail the screen
-        if (R.mol_dmax && R.mol_dmax[bra.mol] == 0.0) continue;
         const int nrows = bra.same ? S::NA : S::NA + S::NB;
         const int c0 = bra.c0;       // the rows hold the molecule's columns c0 .. c0+RW-1
         double* rowA = rows - c0;

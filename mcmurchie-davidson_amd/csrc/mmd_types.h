@@ -52,6 +52,15 @@ struct ClassArgs {
 // takes at most PSLICE bra primitive pairs, a lane at most PSLICE ket ones.
 constexpr int PSLICE = 8;
 
+// Per work unit: an upper bound of the pair factors q_P q_Q over the unit's
+// quartets (rounded up to float) and the molecule of its bra pair, so that a
+// unit whose every quartet fails the density-weighted screen is skipped
+// before anything else of it is read.
+struct UnitBound {
+    float q;
+    int32_t mol;
+};
+
 struct RowArgs {
     int boff, nb;               // bra pairs of the class: pairs[boff .. boff+nb)
     int koff, nk;               // ket pairs of the class
@@ -64,7 +73,8 @@ struct RowArgs {
     int kj;                     // LDS holds the ket-side Coulomb buffer (classes that have one)
     int nshell;
     const double* Dsh;          // shell-block maxima of |dP| (nshell x nshell)
-    const double* mol_dmax;     // batches: max |dP| of each molecule, whose units are skipped at 0; or null
+    const double* mol_dmax;     // batches: max |dP| of each molecule; or null
+    const UnitBound* ubound;    // with mol_dmax: units with 4 q max|dP| < tol are skipped
     double tier_lo, tier_hi;    // this launch takes quartets with tier_lo <= bound < tier_hi
     int* row_counter;           // next work unit (zeroed before the launch)
     unsigned long long* stat_nquart;   // optional statistics

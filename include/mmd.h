@@ -334,8 +334,10 @@ int mmd_dgemm(mmd_ctx *ctx, int transA, int transB, int M, int N, int K, double 
               const double *B, int ldb, double beta, double *C, int ldc, int on_device);
 
 /*
- * Multilayer perceptron of the learned initial density (no counterpart in
- * the reference): tanh hidden layers, linear output layer, FP32 HIP kernels.
+ * Multilayer perceptron of the learned initial densities (no counterpart in
+ * the reference): tanh (default) or SiLU hidden layers
+ * (mmd_mlp_set_activation: 0 tanh, 1 SiLU), linear output layer, FP32 HIP
+ * kernels.
  *   dims            : nlayer + 1 widths (input, hidden..., output)
  *   params          : flat float32, per layer W (fan_in x fan_out, row
  *                     major) then b (fan_out); mmd_mlp_nparam gives the size
@@ -351,6 +353,7 @@ typedef struct mmd_mlp mmd_mlp;
 int mmd_mlp_create(mmd_mlp **out, int nlayer, const int32_t *dims, int device_id);
 void mmd_mlp_destroy(mmd_mlp *mlp);
 int64_t mmd_mlp_nparam(mmd_mlp *mlp);
+int mmd_mlp_set_activation(mmd_mlp *mlp, int activation);
 int mmd_mlp_set_params(mmd_mlp *mlp, const float *params);
 int mmd_mlp_get_params(mmd_mlp *mlp, float *params);
 int mmd_mlp_forward(mmd_mlp *mlp, int n, const float *X, float *Y);

@@ -1,5 +1,5 @@
 // Multilayer perceptron of the learned initial density (no counterpart in
-// the reference): tanh hidden layers, linear output, mean-squared-error loss
+// the reference): tanh or SiLU hidden layers, linear output, mean-squared-error loss
 // and Adam, all in FP32 HIP kernels for gfx950.  The layer sizes are small
 // (tens of units) and the batch is large (thousands of monomers), so each
 // dense layer is a lane-per-output kernel over a batch tile; weight
@@ -75,9 +75,16 @@ __global__ void mlp_gather(const float* __restrict__ X, const int* __restrict__ 
     out[t] = X[(size_t)perm[i] * d + j];
 }
 
-// out[i, j] = act(bias[j] + sum_k H[i, k] W[k, j])
+// Activations of the hidden layers
+constexpr int ACT_LINEAR = 0, ACT_TANH = 1, ACT_SILU = 2;
+
+__device__ __forceinline__ float sigmoidf(float z) { return 1.f / (1.f + expf(-z)); }
+
+// out[i, j] = act(z), z = bias[j] + sum_k H[i, k] W[k, j]; zout (optional)
+// keeps z for the backward pass of SiLU
 __global__ void mlp_dense_fwd(const float* __restrict__ H, const float* __restrict__ W,
-                              const float* __restrict__ bias, int B, int a, int b, int act, float* __restrict__ out)
+                              const float* __restrict__ bias, int B, int a, int b, int act, float* __restrict__ out,
+                              float* __restrict__ zout)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= B * b) return;
@@ -85,7 +92,8 @@ __global__ void mlp_dense_fwd(const float* __restrict__ H, const float* __restri
     float s = bias[j];
     const float* h = H + (size_t)i * a;
     for (int k = 0; k < a; ++k) s = fmaf(h[k], W[(size_t)k * b + j], s);
-    out[t] = act ? tanhf(s) : s;
+    if (zout) zout[t] = s;
+    out[t] = act == ACT_TANH ? tanhf(s) : (act == ACT_SILU ? s * sigmoidf(s) : s);
 }
 
 // dY = scale * 2 (Y - T); loss += scale * sum (Y - T)^2
@@ -136,9 +144,11 @@ __global__ void mlp_dense_bwd_w(const float* __restrict__ H, const float* __rest
     }
 }
 
-// dZprev[i, k] = (sum_j dZ[i, j] W[k, j]) * (1 - H[i, k]^2), H = tanh output
+// dZprev[i, k] = (sum_j dZ[i, j] W[k, j]) * act'(z[i, k]): 1 - H^2 for tanh
+// (H its output), sigma(z) (1 + z (1 - sigma(z))) for SiLU (Z its input)
 __global__ void mlp_dense_bwd_x(const float* __restrict__ dZ, const float* __restrict__ W,
-                                const float* __restrict__ H, int B, int a, int b, float* __restrict__ dprev)
+                                const float* __restrict__ H, const float* __restrict__ Z, int act, int B, int a,
+                                int b, float* __restrict__ dprev)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= B * a) return;
@@ -147,8 +157,13 @@ __global__ void mlp_dense_bwd_x(const float* __restrict__ dZ, const float* __res
     const float* dz = dZ + (size_t)i * b;
     const float* w = W + (size_t)k * b;
     for (int j = 0; j < b; ++j) s = fmaf(dz[j], w[j], s);
-    const float h = H[t];
-    dprev[t] = s * (1.f - h * h);
+    if (act == ACT_SILU) {
+        const float z = Z[t], sg = sigmoidf(z);
+        dprev[t] = s * sg * (1.f + z * (1.f - sg));
+    } else {
+        const float h = H[t];
+        dprev[t] = s * (1.f - h * h);
+    }
 }
 
 __global__ void mlp_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
@@ -196,8 +211,10 @@ struct mmd_mlp {
     Buf<float> X, Y;             // training data on the device
     Buf<int> perm;
     std::vector<Buf<float>> act; // activations per layer (act[0] = input batch)
+    std::vector<Buf<float>> pre; // SiLU: pre-activations of the hidden layers
     std::vector<Buf<float>> dz;  // gradients w.r.t. pre-activations per layer
     int adam_t = 0;
+    int hidden_act = ACT_TANH;   // activation of the hidden layers
 };
 
 namespace {
@@ -207,6 +224,8 @@ int mlp_reserve(mmd_mlp* h, int B)
     const int L = (int)h->dims.size() - 1;
     for (int l = 0; l <= L; ++l) MLP_TRY(h->act[l].reserve((size_t)B * h->dims[l]));
     for (int l = 1; l <= L; ++l) MLP_TRY(h->dz[l].reserve((size_t)B * h->dims[l]));
+    if (h->hidden_act == ACT_SILU)
+        for (int l = 1; l < L; ++l) MLP_TRY(h->pre[l].reserve((size_t)B * h->dims[l]));
     return MMD_OK;
 }
 
@@ -216,9 +235,11 @@ int mlp_forward_batch(mmd_mlp* h, int B)
     const int L = (int)h->dims.size() - 1;
     for (int l = 1; l <= L; ++l) {
         const int a = h->dims[l - 1], b = h->dims[l];
+        const bool hidden = l < L;
         hipLaunchKernelGGL(mlp_dense_fwd, dim3(blocks((size_t)B * b)), dim3(256), 0, h->st, h->act[l - 1].p,
-                           h->param.p + h->woff[l - 1], h->param.p + h->boff[l - 1], B, a, b, l < L ? 1 : 0,
-                           h->act[l].p);
+                           h->param.p + h->woff[l - 1], h->param.p + h->boff[l - 1], B, a, b,
+                           hidden ? h->hidden_act : ACT_LINEAR, h->act[l].p,
+                           hidden && h->hidden_act == ACT_SILU ? h->pre[l].p : nullptr);
         MLP_TRY(hipGetLastError());
     }
     return MMD_OK;
@@ -240,7 +261,9 @@ int mlp_backward_batch(mmd_mlp* h, int B, const float* T, float scale)
         MLP_TRY(hipGetLastError());
         if (l > 1) {
             hipLaunchKernelGGL(mlp_dense_bwd_x, dim3(blocks((size_t)B * a)), dim3(256), 0, h->st, h->dz[l].p,
-                               h->param.p + h->woff[l - 1], h->act[l - 1].p, B, a, b, h->dz[l - 1].p);
+                               h->param.p + h->woff[l - 1], h->act[l - 1].p,
+                               h->hidden_act == ACT_SILU ? h->pre[l - 1].p : nullptr, h->hidden_act, B, a, b,
+                               h->dz[l - 1].p);
             MLP_TRY(hipGetLastError());
         }
     }
@@ -281,6 +304,7 @@ int mmd_mlp_create(mmd_mlp** out, int nlayer, const int32_t* dims, int device_id
     }
     h->nparam = off;
     h->act.resize(nlayer + 1);
+    h->pre.resize(nlayer + 1);
     h->dz.resize(nlayer + 1);
     hipError_t e = hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking);
     if (e == hipSuccess) e = h->param.reserve(off);
@@ -307,6 +331,14 @@ void mmd_mlp_destroy(mmd_mlp* h)
 }
 
 int64_t mmd_mlp_nparam(mmd_mlp* h) { return h ? (int64_t)h->nparam : (int64_t)MMD_ERR_ARG; }
+
+int mmd_mlp_set_activation(mmd_mlp* h, int activation)
+{
+    if (!h) return mlp_fail(MMD_ERR_ARG, "null perceptron");
+    if (activation != 0 && activation != 1) return mlp_fail(MMD_ERR_ARG, "activation must be 0 (tanh) or 1 (SiLU)");
+    h->hidden_act = activation == 1 ? ACT_SILU : ACT_TANH;
+    return MMD_OK;
+}
 
 int mmd_mlp_set_params(mmd_mlp* h, const float* params)
 {

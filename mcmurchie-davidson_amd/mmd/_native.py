@@ -116,6 +116,8 @@ lib.mmd_mlp_destroy.restype = None
 lib.mmd_mlp_destroy.argtypes = [_vp]
 lib.mmd_mlp_nparam.restype = _c_i64
 lib.mmd_mlp_nparam.argtypes = [_vp]
+lib.mmd_mlp_set_activation.restype = _c_int
+lib.mmd_mlp_set_activation.argtypes = [_vp, _c_int]
 lib.mmd_mlp_set_params.restype = _c_int
 lib.mmd_mlp_set_params.argtypes = [_vp, _vp]
 lib.mmd_mlp_get_params.restype = _c_int
@@ -135,7 +137,7 @@ EXPORTED = [
     "mmd_last_gradient_ms", "mmd_set_precision", "mmd_fock_tiers",
     "mmd_set_streams", "mmd_set_lds_limit", "mmd_overlap_powers", "mmd_dgemm", "mmd_set_molecules",
     "mmd_batch_rhf", "mmd_scf_begin", "mmd_scf_step", "mmd_scf_density", "mmd_scf_get", "mmd_ao2mo",
-    "mmd_mlp_create", "mmd_mlp_destroy", "mmd_mlp_nparam", "mmd_mlp_set_params", "mmd_mlp_get_params",
+    "mmd_mlp_create", "mmd_mlp_destroy", "mmd_mlp_nparam", "mmd_mlp_set_activation", "mmd_mlp_set_params", "mmd_mlp_get_params",
     "mmd_mlp_forward", "mmd_mlp_loss_grad", "mmd_mlp_train",
 ]
 
@@ -461,15 +463,20 @@ class Context:
 
 
 class MLP:
-    """The learned guess's perceptron on the GPU (mmd_mlp_* of include/mmd.h):
-    tanh hidden layers, linear output, FP32."""
+    """The learned guesses' perceptron on the GPU (mmd_mlp_* of include/mmd.h):
+    tanh or SiLU hidden layers (activation 'tanh' or 'silu'), linear output,
+    FP32."""
 
-    def __init__(self, dims, device=0):
+    def __init__(self, dims, device=0, activation="tanh"):
         self.dims = [int(d) for d in dims]
+        if activation not in ("tanh", "silu"):
+            raise ValueError("activation must be 'tanh' or 'silu'")
         h = _vp()
         d = np.ascontiguousarray(self.dims, dtype=np.int32)
         check(lib.mmd_mlp_create(ctypes.byref(h), len(self.dims) - 1, _ptr(d), device))
         self._h = h
+        self.activation = activation
+        check(lib.mmd_mlp_set_activation(h, 1 if activation == "silu" else 0))
         self.nparam = int(lib.mmd_mlp_nparam(h))
 
     def close(self):
@@ -523,8 +530,9 @@ def mlp_forward(ctx, model, X):
     """Forward pass of a mmd.learned.Model on the GPU of ``ctx``."""
     dev = getattr(ctx, "device", 0)
     net = getattr(model, "_gpu", None)
-    if net is None or net.dims != model.dims:
-        net = MLP(model.dims, dev)
+    act = getattr(model, "activation", "tanh")
+    if net is None or net.dims != model.dims or net.activation != act:
+        net = MLP(model.dims, dev, activation=act)
         net.set_params(model.flat())
         model._gpu = net
     return net.forward(X)

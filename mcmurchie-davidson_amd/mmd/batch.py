@@ -299,6 +299,10 @@ def _device_group(ctx, geometries, idx, basis, guess, fp32_below, conver, acc2e,
         from mmd.learned import batch_learned_densities
         dens = batch_learned_densities([(a[0], a[1], a[3], 2 * a[7]) for a in arrs], basis, guess_model, ctx)
         P0 = np.concatenate([d.ravel() for d in dens])
+    elif guess == "pair":
+        from mmd.pairguess import batch_pair_densities
+        dens = batch_pair_densities([(a[0], a[1], a[3], 2 * a[7]) for a in arrs], basis, guess_model, ctx)
+        P0 = np.concatenate([d.ravel() for d in dens])
     elif guess == "sad":
         from mmd.learned import _element_density
         blocks = []
@@ -346,8 +350,9 @@ def run_batch_device(geometries, basis="6-31g", group=64, guess="core", fp32_bel
                      streams=2, class_streams=None, sort_by_size=True):
     """Direct RHF energies of all geometries, `group` molecules at a time in
     one block-diagonal basis on the GPU (mmd_batch_rhf).  guess: 'core' (the
-    reference's), 'sad' or 'learned' (initial densities formed for the whole
-    group on the host side, the network on the GPU).  fp32_below: the FP32
+    reference's), 'sad', 'learned' (the monomer network) or 'pair' (the
+    atom-pair networks); initial densities formed for the whole group on the
+    host side, the networks on the GPU.  fp32_below: the FP32
     tier of the Fock builds.  streams: groups in flight at once, each on its
     own context (stream), so that one group's host-side set-up overlaps
     another's device work; with ctx given, that context alone.
@@ -360,11 +365,14 @@ def run_batch_device(geometries, basis="6-31g", group=64, guess="core", fp32_bel
     P with keep_density)."""
     if group < 1:
         raise ValueError("group must be >= 1")
-    if guess not in ("core", "sad", "learned"):
-        raise ValueError("guess must be 'core', 'sad' or 'learned'")
+    if guess not in ("core", "sad", "learned", "pair"):
+        raise ValueError("guess must be 'core', 'sad', 'learned' or 'pair'")
     if guess == "learned":   # the network's weights once, before any worker needs them
         from mmd.learned import default_model
         guess_model = guess_model or default_model()
+    elif guess == "pair":
+        from mmd.pairguess import default_model as default_pair_model
+        guess_model = guess_model or default_pair_model()
     order = list(range(len(geometries)))
     if sort_by_size:
         natoms = [sum(1 for ln in g.split("\n")[1:] if ln.strip()) for g in geometries]

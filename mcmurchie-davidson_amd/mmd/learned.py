@@ -349,15 +349,17 @@ def make_dataset(geometries, basis="6-31g", workers=8):
     return np.concatenate(X), np.concatenate(Y)
 
 
-def train_model(X, Y, hidden=(64, 64), epochs=200, batch=512, lr=2e-3, seed=0, device=0):
+def train_model(X, Y, hidden=(64, 64), epochs=200, batch=512, lr=2e-3, seed=0, device=0, activation="tanh"):
     """Fit the perceptron to (X, Y) with the GPU trainer; returns the Model
-    and the per-epoch training loss (normalised units)."""
+    and the per-epoch training loss (normalised units).  activation: of the
+    hidden layers, 'tanh' or 'silu'."""
     from mmd import _native
     x_mean, x_std = X.mean(axis=0), X.std(axis=0) + 1e-8
     y_std = float(Y.std()) + 1e-12
     dims = [X.shape[1]] + list(hidden) + [Y.shape[1]]
     model = Model.init(dims, x_mean, x_std, y_std, seed=seed)
-    net = _native.MLP(dims, device)
+    model.activation = activation
+    net = _native.MLP(dims, device, activation=activation)
     net.set_params(model.flat())
     Xn = ((X - x_mean) / x_std).astype(np.float32)
     Yn = (Y / y_std).astype(np.float32)

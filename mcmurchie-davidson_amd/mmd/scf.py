@@ -85,9 +85,11 @@ class SCF(object):
         guess='sad' starts from the superposition of atomic densities
         (mmd.guess.sad_density): F = H + G(P_sad), and the first incremental
         build is G(P_1 - P_sad); guess='learned' starts from SAD corrected
-        per water monomer by the trained perceptron (mmd.learned)."""
-        if guess not in ('core', 'sad', 'learned'):
-            raise ValueError("guess must be 'core', 'sad' or 'learned'")
+        per water monomer by the trained perceptron (mmd.learned);
+        guess='pair' from SAD with the atom-pair blocks of the atom-pair
+        networks (mmd.pairguess)."""
+        if guess not in ('core', 'sad', 'learned', 'pair'):
+            raise ValueError("guess must be 'core', 'sad', 'learned' or 'pair'")
         gemm_threads, self._eigh_threads = _thread_plan(self.nbasis)
         try:
             with _blas_threads(gemm_threads):
@@ -122,10 +124,13 @@ class SCF(object):
 
         self.P = self.P_old
         self.F = self.Core.astype('complex')
-        if guess in ('sad', 'learned'):
+        if guess in ('sad', 'learned', 'pair'):
             if guess == 'sad':
                 from mmd.guess import sad_density
                 self.P = sad_density(self).astype('complex')
+            elif guess == 'pair':
+                from mmd.pairguess import pair_density
+                self.P = pair_density(self, getattr(self, 'pair_model', None)).astype('complex')
             else:
                 from mmd.learned import learned_density
                 self.P = learned_density(self, getattr(self, 'guess_model', None)).astype('complex')

@@ -394,20 +394,22 @@ def test_fp32_tier_energy_within_microhartree(native):
 # and backward against the numpy restatement (FP32: 1e-4 relative), and
 # training by the HIP Adam loop.
 # --------------------------------------------------------------------------
-def test_mlp_forward_and_gradient_vs_numpy(native):
+@pytest.mark.parametrize("activation", ["tanh", "silu"])
+def test_mlp_forward_and_gradient_vs_numpy(native, activation):
     import mlp_ref
     dims = [15, 32, 24, 91]
-    net = native.MLP(dims)
+    net = native.MLP(dims, activation=activation)
     rng = np.random.default_rng(8)
     flat = (rng.standard_normal(net.nparam) * 0.3).astype(np.float32)
     net.set_params(flat)
     assert np.array_equal(net.get_params(), flat)
     X = rng.standard_normal((300, 15)).astype(np.float32)
     Y = rng.standard_normal((300, 91)).astype(np.float32)
-    ref = mlp_ref.forward(flat.astype(np.float64), dims, X.astype(np.float64))[-1]
+    ref = mlp_ref.forward(flat.astype(np.float64), dims, X.astype(np.float64), activation)[-1]
     assert np.max(np.abs(net.forward(X) - ref)) < 1e-4 * np.max(np.abs(ref))
     loss, g = net.loss_grad(X, Y)
-    loss_ref, g_ref = mlp_ref.loss_grad(flat.astype(np.float64), dims, X.astype(np.float64), Y.astype(np.float64))
+    loss_ref, g_ref = mlp_ref.loss_grad(flat.astype(np.float64), dims, X.astype(np.float64), Y.astype(np.float64),
+                                        activation)
     assert abs(loss - loss_ref) < 1e-5 * loss_ref
     assert np.max(np.abs(g - g_ref)) < 1e-4 * np.max(np.abs(g_ref))
     with pytest.raises(ValueError):
@@ -451,6 +453,60 @@ def test_learned_guess_gain_on_heldout(native):
             it[guess] += m.iterations
     print("iterations over %d held-out clusters: %s" % (len(geoms), it))
     assert it["learned"] < it["sad"]
+
+
+def _random_pair_model(seed):
+    from mmd import pairguess as pg
+    from mmd.learned import Model
+    layouts = pg.element_layouts("6-31g")
+    nets = {}
+    rng = np.random.default_rng(seed)
+    for zi in pg.ELEMENTS:
+        for zj in pg.ELEMENTS:
+            nout = pg.PairShape(layouts[zi][1], layouts[zj][1], layouts[zi][0], layouts[zj][0]).size
+            dims = [pg.NRBF] + list(pg.HIDDEN) + [nout]
+            m = Model.init(dims, np.zeros(pg.NRBF), np.ones(pg.NRBF), 0.05, seed=int(rng.integers(1 << 30)))
+            m.activation = "silu"
+            nets[(zi, zj)] = m
+    return pg.PairModel(nets, layouts)
+
+
+def test_pair_guess_rotates_with_the_cluster(native):
+    """Atom-pair guess on the GPU networks (random weights): the density of a
+    rotated cluster is the rotated density (p functions of every atom turned
+    by the rotation), and it is symmetric."""
+    from mmd import pairguess as pg
+    from mmd.batch import _molecule_arrays
+    from mmd.learned import _monomer_rotation
+    model = _random_pair_model(1)
+    g = water_cluster(3, 7.0, 5)
+    Z, xyz, _, lmn, _, _, _, nocc, _ = _molecule_arrays(g, "6-31g")
+    th = 0.7
+    Q = np.array([[np.cos(th), -np.sin(th), 0.0], [np.sin(th), np.cos(th), 0.0], [0.0, 0.0, 1.0]])
+    Q = Q @ np.array([[1.0, 0.0, 0.0], [0.0, np.cos(0.4), -np.sin(0.4)], [0.0, np.sin(0.4), np.cos(0.4)]])
+    P1, P2 = pg.batch_pair_densities([(Z, xyz, lmn, 2 * nocc), (Z, xyz @ Q.T, lmn, 2 * nocc)], "6-31g", model)
+    U = np.kron(np.eye(3), _monomer_rotation(lmn[:13], Q[None])[0])
+    assert np.max(np.abs(P1 - P1.T)) < 1e-14
+    assert np.max(np.abs(P1[:9, 9:11])) > 1e-3           # O-H blocks filled
+    assert np.max(np.abs(U @ P1 @ U.T - P2)) < 2e-5 * np.max(np.abs(P1))
+
+
+def test_pair_guess_converges_to_the_core_start_energy(native):
+    """The trained atom-pair networks (mmd/data/pair_guess_631g.npz) as the
+    start of the per-molecule and the batched SCF: same energies as the core
+    start."""
+    from mmd.batch import run_batch_device
+    from mmd.molecule import Molecule
+    geoms = [water_cluster(k, 2.0 + 1.4 * k, seed=17 + k, perturb=True) for k in (2, 4)]
+    batch = run_batch_device(geoms, "6-31g", guess="pair", conver=1e-10)
+    for g, b in zip(geoms, batch):
+        core = Molecule(g, "6-31g")
+        core.RHF(doPrint=False, direct=True, conver=1e-10)
+        m = Molecule(g, "6-31g")
+        m.RHF(doPrint=False, direct=True, conver=1e-10, guess="pair")
+        assert m.is_converged and b["converged"]
+        assert abs(m.energy.real - core.energy.real) < 1e-8
+        assert abs(b["energy"] - core.energy.real) < 1e-8
 
 
 # --------------------------------------------------------------------------

@@ -191,3 +191,67 @@ def test_batched_learned_guess_matches_per_molecule(monkeypatch):
         corr = (X.astype(np.float64) @ Wmap) * float(model.y_std)
         ref = learned.assemble(mol, base, corr)
         assert np.allclose(P, ref, rtol=0, atol=1e-10)
+
+
+@pytest.mark.parametrize("activation", ["tanh", "silu"])
+def test_mlp_reference_gradient_matches_finite_differences(activation):
+    """The numpy perceptron that the GPU trainer is checked against: its
+    analytic gradient against central differences (both activations)."""
+    import mlp_ref
+    dims = [5, 7, 6, 3]
+    rng = np.random.default_rng(3)
+    n = sum(a * b + b for a, b in zip(dims[:-1], dims[1:]))
+    flat = rng.standard_normal(n) * 0.5
+    X = rng.standard_normal((11, 5))
+    Y = rng.standard_normal((11, 3))
+    _, g = mlp_ref.loss_grad(flat, dims, X, Y, activation)
+    h = 1e-6
+    fd = np.empty(n)
+    for k in range(n):
+        e = np.zeros(n)
+        e[k] = h
+        fd[k] = (mlp_ref.loss_grad(flat + e, dims, X, Y, activation)[0]
+                 - mlp_ref.loss_grad(flat - e, dims, X, Y, activation)[0]) / (2 * h)
+    assert np.max(np.abs(g - fd)) < 1e-7 * max(1.0, np.max(np.abs(fd)))
+
+
+def _rotation(seed):
+    rng = np.random.default_rng(seed)
+    q, r = np.linalg.qr(rng.standard_normal((3, 3)))
+    q *= np.sign(np.diag(r))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
+def test_pair_guess_invariants_round_trip_and_rotation():
+    """Atom-pair guess geometry (mmd.pairguess): the bond-frame invariants of
+    a density are unchanged when the cluster and the density are rotated
+    together, and blocks built from invariants give back the invariants."""
+    from mmd import pairguess as pg
+    from mmd.learned import _monomer_rotation
+    lay = {8: (9, ([0, 1, 5], [2, 6])), 1: (2, ([0, 1], []))}
+    rng = np.random.default_rng(4)
+    Z = np.array([8, 1, 1, 8, 1, 1])
+    xyz = rng.standard_normal((6, 3)) * 2.5
+    n = 26
+    A = rng.standard_normal((n, n))
+    P = A + A.T
+    data = pg.training_pairs(Z, xyz, P, lay)
+    assert set(data) == {(8, 8), (8, 1), (1, 8), (1, 1)}
+    sizes = {(8, 8): 29, (8, 1): 10, (1, 8): 10, (1, 1): 4}
+    for k, (F, Y) in data.items():
+        assert F.shape[1] == pg.NRBF and Y.shape[1] == sizes[k]
+        sh = pg.PairShape(lay[k[0]][1], lay[k[1]][1], lay[k[0]][0], lay[k[1]][0])
+        assert np.allclose(sh.invariants(sh.blocks(Y)), Y, atol=1e-14)
+    # rotate the cluster and the density (p triples of every atom by Q)
+    Q = _rotation(5)
+    lmn = np.array([[0, 0, 0]] * 2 + [[1, 0, 0], [0, 1, 0], [0, 0, 1]] + [[0, 0, 0]]
+                   + [[1, 0, 0], [0, 1, 0], [0, 0, 1]] + [[0, 0, 0]] * 4)
+    U = _monomer_rotation(lmn[:13], Q[None])[0]
+    Ufull = np.kron(np.eye(2), U)
+    P2 = Ufull @ P @ Ufull.T
+    data2 = pg.training_pairs(Z, xyz @ Q.T, P2, lay)
+    for k in data:
+        assert np.allclose(data[k][0], data2[k][0], atol=1e-12)
+        assert np.allclose(data[k][1], data2[k][1], atol=1e-10)

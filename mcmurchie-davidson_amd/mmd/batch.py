@@ -262,67 +262,96 @@ def _element_template(basis, Z):
     return t
 
 
-def _molecule_arrays(geometry, basis):
-    """(Z, xyz bohr, origin, lmn, nprim, exps, weights, nocc, E_nuc) of one
-    geometry string, as Molecule(geometry, basis) would build them."""
+def _group_arrays(geometries, basis):
+    """The arrays of many geometry strings at once, as Molecule(geometry,
+    basis) would build them: per molecule (Z, xyz bohr, nocc, E_nuc, first
+    function, function count), and the basis of all of them concatenated
+    (origin, lmn, nprim, exps, weights), each built in one pass over the
+    group's atoms from the per-element templates."""
     from mmd.molecule import BOHR_ANGSTROM, _SYMBOLS
-    rows = [ln.split() for ln in geometry.split("\n") if ln.strip()]
-    charge = int(rows[0][0])
-    Z = np.array([_SYMBOLS.index(r[0]) for r in rows[1:]], dtype=np.int64)
-    xyz = np.array([[float(x) for x in r[1:4]] for r in rows[1:]]) / BOHR_ANGSTROM
-    parts = [_element_template(basis, int(z)) for z in Z]
+    mols, Zs, Xs = [], [], []
+    kinds = {}
+    for k, g in enumerate(geometries):
+        rows = [ln.split() for ln in g.split("\n") if ln.strip()]
+        Z = np.array([_SYMBOLS.index(r[0]) for r in rows[1:]], dtype=np.int64)
+        xyz = np.array([r[1:4] for r in rows[1:]], dtype=np.float64) / BOHR_ANGSTROM
+        mols.append([Z, xyz, (int(Z.sum()) - int(rows[0][0])) // 2, 0.0])
+        kinds.setdefault(tuple(Z), []).append(k)
+        Zs.append(Z)
+        Xs.append(xyz)
+    # nuclear repulsion, molecules of one composition at once
+    for Zt, ks in kinds.items():
+        Z = np.array(Zt, dtype=np.float64)
+        X = np.stack([mols[k][1] for k in ks])
+        iu = np.triu_indices(len(Z), 1)
+        d = np.linalg.norm(X[:, iu[0]] - X[:, iu[1]], axis=-1)
+        e = ((Z[iu[0]] * Z[iu[1]])[None] / d).sum(axis=1)
+        for k, v in zip(ks, e):
+            mols[k][3] = float(v)
+    Zall = np.concatenate(Zs)
+    T = {int(z): _element_template(basis, int(z)) for z in np.unique(Zall)}
+    parts = [T[int(z)] for z in Zall]
+    nbf_atom = np.array([len(p[1]) for p in parts])
     lmn = np.concatenate([p[0] for p in parts])
     nprim = np.concatenate([p[1] for p in parts])
     exps = np.concatenate([p[2] for p in parts])
     w = np.concatenate([p[3] for p in parts])
-    origin = np.concatenate([np.repeat(xyz[i:i + 1], len(p[1]), axis=0) for i, p in enumerate(parts)])
-    d = np.linalg.norm(xyz[:, None, :] - xyz[None, :, :], axis=-1)
-    iu = np.triu_indices(len(Z), 1)
-    enuc = float(np.sum(np.outer(Z, Z)[iu] / d[iu]))
-    nocc = (int(Z.sum()) - charge) // 2
-    return Z, xyz, origin, lmn, nprim, exps, w, nocc, enuc
+    origin = np.repeat(np.concatenate(Xs), nbf_atom, axis=0)
+    f0, a0 = 0, 0
+    for m in mols:
+        n = int(nbf_atom[a0:a0 + len(m[0])].sum())
+        m += [f0, n]
+        f0 += n
+        a0 += len(m[0])
+    return mols, (origin, lmn, nprim, exps, w)
+
+
+def _molecule_arrays(geometry, basis):
+    """(Z, xyz bohr, origin, lmn, nprim, exps, weights, nocc, E_nuc) of one
+    geometry string, as Molecule(geometry, basis) would build them."""
+    (m,), (origin, lmn, nprim, exps, w) = _group_arrays([geometry], basis)
+    return m[0], m[1], origin, lmn, nprim, exps, w, m[2], m[3]
 
 
 def _device_group(ctx, geometries, idx, basis, guess, fp32_below, conver, acc2e, use_graph, keep_density,
                   guess_model, out):
     """One group of run_batch_device on ctx: results into out[i] for i in idx."""
-    arrs = [_molecule_arrays(geometries[i], basis) for i in idx]
-    ctx.set_basis(np.concatenate([a[2] for a in arrs]), np.concatenate([a[3] for a in arrs]),
-                  np.concatenate([a[4] for a in arrs]), np.concatenate([a[5] for a in arrs]),
-                  np.concatenate([a[6] for a in arrs]))
-    ctx.set_atoms(np.concatenate([a[0] for a in arrs]).astype(np.float64),
-                  np.concatenate([a[1] for a in arrs]))
-    ctx.set_molecules([len(a[4]) for a in arrs], [len(a[0]) for a in arrs])
+    mols, (origin, lmn, nprim, exps, w) = _group_arrays([geometries[i] for i in idx], basis)
+    ctx.set_basis(origin, lmn, nprim, exps, w)
+    ctx.set_atoms(np.concatenate([m[0] for m in mols]).astype(np.float64), np.concatenate([m[1] for m in mols]))
+    ctx.set_molecules([m[5] for m in mols], [len(m[0]) for m in mols])
     ctx.set_precision(fp32_below)
     P0 = None
+    # per molecule (Z, xyz, lmn of its functions, electrons) for the guesses
+    gm = [(m[0], m[1], lmn[m[4]:m[4] + m[5]], 2 * m[2]) for m in mols]
     if guess == "learned":
         from mmd.learned import batch_learned_densities
-        dens = batch_learned_densities([(a[0], a[1], a[3], 2 * a[7]) for a in arrs], basis, guess_model, ctx)
+        dens = batch_learned_densities(gm, basis, guess_model, ctx)
         P0 = np.concatenate([d.ravel() for d in dens])
     elif guess == "pair":
         from mmd.pairguess import batch_pair_densities
-        dens = batch_pair_densities([(a[0], a[1], a[3], 2 * a[7]) for a in arrs], basis, guess_model, ctx)
+        dens = batch_pair_densities(gm, basis, guess_model, ctx)
         P0 = np.concatenate([d.ravel() for d in dens])
     elif guess == "sad":
         from mmd.learned import _element_density
         blocks = []
-        for a in arrs:
-            parts = [_element_density(int(z), basis) for z in a[0]]
+        for Z, _, _, nelec in gm:
+            parts = [_element_density(int(z), basis) for z in Z]
             n = sum(len(b) for b in parts)
             P = np.zeros((n, n))
             o = 0
             for b in parts:
                 P[o:o + len(b), o:o + len(b)] = b
                 o += len(b)
-            if a[0].sum() != 2 * a[7]:
-                P *= 2 * a[7] / a[0].sum()
+            if Z.sum() != nelec:
+                P *= nelec / Z.sum()
             blocks.append(P.ravel())
         P0 = np.concatenate(blocks)
-    res = ctx.batch_rhf([a[7] for a in arrs], [a[8] for a in arrs], P0=P0, conver=conver, acc2e=acc2e,
+    res = ctx.batch_rhf([m[2] for m in mols], [m[3] for m in mols], P0=P0, conver=conver, acc2e=acc2e,
                         use_graph=use_graph, keep_density=keep_density)
     o = 0
     for k, i in enumerate(idx):
-        n = len(arrs[k][4])
+        n = mols[k][5]
         r = {"energy": float(res[0][k]), "iterations": int(res[1][k]), "converged": bool(res[2][k]),
              "nbasis": n}
         if keep_density:
@@ -388,7 +417,9 @@ def run_batch_device(geometries, basis="6-31g", group=64, guess="core", fp32_bel
         for idx in groups:
             _device_group(ctxs[0], geometries, idx, *args)
         return out
-    todo = list(reversed(groups))
+    # groups are taken from the end of the list: with sort_by_size the largest
+    # molecules first, so that the groups that finish last are the short ones
+    todo = list(groups) if sort_by_size else list(reversed(groups))
     lock = threading.Lock()
     errors = []
 

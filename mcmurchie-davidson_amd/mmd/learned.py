@@ -231,42 +231,51 @@ def _element_density(Z, basis):
         return atomic_density(_Mol, _Atom)
 
 
+def _features_stack(Z, X):
+    """batch_features() of M clusters of the same composition at once: Z
+    (n_atom,), X (M, n_atom, 3) bohr.  Returns features (M, n_monomer, 15)
+    and local frames (M, n_monomer, 3, 3)."""
+    Z = np.asarray(Z)
+    if len(Z) % 3 or np.any(Z.reshape(-1, 3) != (8, 1, 1)):
+        raise ValueError("the learned guess handles water clusters (O, H, H per monomer)")
+    X = np.asarray(X, dtype=float)
+    M, na = X.shape[0], len(Z)
+    nm = na // 3
+    O, H1, H2 = X[:, 0::3], X[:, 1::3], X[:, 2::3]
+    v1, v2 = H1 - O, H2 - O
+    r1 = np.linalg.norm(v1, axis=2)
+    r2 = np.linalg.norm(v2, axis=2)
+    u1, u2 = v1 / r1[..., None], v2 / r2[..., None]
+    x = u1 + u2
+    x /= np.linalg.norm(x, axis=2)[..., None]
+    y = u1 - np.sum(u1 * x, axis=2)[..., None] * x
+    y /= np.linalg.norm(y, axis=2)[..., None]
+    z = np.cross(x, y)
+    R = np.stack([x, y, z], axis=3)
+    F = np.zeros((M, nm, 15))
+    F[..., 0], F[..., 1] = r1, r2
+    F[..., 2] = np.arccos(np.clip(np.sum(v1 * v2, axis=2) / (r1 * r2), -1.0, 1.0))
+    if nm > 1:
+        q = np.where(Z == 8, Q_O, Q_H)
+        d = X[:, :, None, :] - X[:, None, :, :]
+        r = np.linalg.norm(d, axis=3)
+        mon = np.arange(na) // 3
+        other = (mon[:, None] != mon[None, :])[None]
+        rinv = np.where(other, 1.0 / np.where(other, r, 1.0), 0.0)
+        pot = (q[None, None, :] * rinv).sum(axis=2)
+        E = np.einsum("Mij,Mijx->Mix", q[None, None, :] * rinv ** 3, d)
+        F[..., 3:6] = pot.reshape(M, nm, 3)
+        El = np.einsum("Mnxa,Mnkx->Mnka", R, E.reshape(M, nm, 3, 3))
+        F[..., 6:15] = El.reshape(M, nm, 9)
+    return F, R
+
+
 def batch_features(Z, xyz):
     """features() of one water cluster from arrays: atomic numbers Z and
     coordinates xyz (bohr) in O, H, H order per monomer.  Returns
     (features (n_monomer, 15), local frames R (n_monomer, 3, 3))."""
-    Z = np.asarray(Z)
-    if len(Z) % 3 or np.any(Z.reshape(-1, 3) != (8, 1, 1)):
-        raise ValueError("the learned guess handles water clusters (O, H, H per monomer)")
-    xyz = np.asarray(xyz, dtype=float)
-    nm = len(Z) // 3
-    O, H1, H2 = xyz[0::3], xyz[1::3], xyz[2::3]
-    v1, v2 = H1 - O, H2 - O
-    r1 = np.linalg.norm(v1, axis=1)
-    r2 = np.linalg.norm(v2, axis=1)
-    u1, u2 = v1 / r1[:, None], v2 / r2[:, None]
-    x = u1 + u2
-    x /= np.linalg.norm(x, axis=1)[:, None]
-    y = u1 - np.sum(u1 * x, axis=1)[:, None] * x
-    y /= np.linalg.norm(y, axis=1)[:, None]
-    z = np.cross(x, y)
-    R = np.stack([x, y, z], axis=2)
-    F = np.zeros((nm, 15))
-    F[:, 0], F[:, 1] = r1, r2
-    F[:, 2] = np.arccos(np.clip(np.sum(v1 * v2, axis=1) / (r1 * r2), -1.0, 1.0))
-    if nm > 1:
-        q = np.where(Z == 8, Q_O, Q_H)
-        d = xyz[:, None, :] - xyz[None, :, :]
-        r = np.linalg.norm(d, axis=2)
-        mon = np.arange(len(Z)) // 3
-        other = mon[:, None] != mon[None, :]
-        rinv = np.where(other, 1.0 / np.where(other, r, 1.0), 0.0)
-        pot = (q[None, :] * rinv).sum(axis=1)
-        E = np.einsum("ij,ijx->ix", q[None, :] * rinv ** 3, d)
-        F[:, 3:6] = pot.reshape(nm, 3)
-        El = np.einsum("mxa,mkx->mka", R, E.reshape(nm, 3, 3))
-        F[:, 6:15] = El.reshape(nm, 9)
-    return F, R
+    F, R = _features_stack(Z, np.asarray(xyz, dtype=float)[None])
+    return F[0], R[0]
 
 
 def _monomer_rotation(lmn, R):
@@ -290,45 +299,53 @@ def batch_learned_densities(molecules, basis, model=None, ctx=None, element_dens
     """learned_density() of many water clusters with one forward pass.
 
     molecules: per cluster (Z, xyz bohr, lmn of its functions, electron
-    count).  Returns the half densities, one (n, n) array each.  element_density(Z, basis) gives
+    count).  Returns the half densities, one (n, n) array each.  Clusters of
+    the same composition are handled together (features, frames and the
+    assembly as stacked array operations).  element_density(Z, basis) gives
     the free-atom block (default: the cached SAD atomic density)."""
     from mmd import _native
     model = model or default_model()
     element_density = element_density or _element_density
-    feats, frames = [], []
-    for Z, xyz, _, _ in molecules:
-        F, R = batch_features(Z, xyz)
-        feats.append(F)
-        frames.append(R)
-    X = ((np.concatenate(feats) - model.x_mean) / model.x_std).astype(np.float32)
+    kinds = {}
+    for k, (Z, xyz, lmn, nelec) in enumerate(molecules):
+        kinds.setdefault((tuple(int(z) for z in Z), int(nelec)), []).append(k)
+    stacks = []
+    for (Zt, nelec), ks in kinds.items():
+        F, R = _features_stack(np.array(Zt), np.stack([np.asarray(molecules[k][1], dtype=float) for k in ks]))
+        stacks.append((Zt, nelec, ks, F, R))
+    X = np.concatenate([st[3].reshape(-1, 15) for st in stacks])
+    X = ((X - model.x_mean) / model.x_std).astype(np.float32)
     with _LOCK:   # the network's device buffers are shared
         Y = _native.mlp_forward(ctx, model, X).astype(float) * float(model.y_std)
-    out = []
-    k = 0
-    for (Z, xyz, lmn, nelec), R in zip(molecules, frames):
-        Z = np.asarray(Z)
+    out = [None] * len(molecules)
+    row = 0
+    for Zt, nelec, ks, F, R in stacks:
+        Z = np.array(Zt)
+        M, nm = F.shape[0], F.shape[1]
         blocks = [element_density(int(z), basis) for z in Z]
         sizes = [b.shape[0] for b in blocks]
         n = sum(sizes)
-        P = np.zeros((n, n))
+        P0 = np.zeros((n, n))
         o = 0
         for b in blocks:
-            P[o:o + len(b), o:o + len(b)] = b
+            P0[o:o + len(b), o:o + len(b)] = b
             o += len(b)
         if Z.sum() != nelec:   # sad_density's scaling of a charged cluster
-            P *= nelec / Z.sum()
-        nm = len(Z) // 3
+            P0 *= nelec / Z.sum()
         nb = sizes[0] + sizes[1] + sizes[2]   # functions of one monomer
         iu = np.triu_indices(nb)
-        C = np.zeros((nm, nb, nb))
-        C[:, iu[0], iu[1]] = Y[k:k + nm]
+        C = np.zeros((M * nm, nb, nb))
+        C[:, iu[0], iu[1]] = Y[row:row + M * nm]
+        row += M * nm
         C = C + np.transpose(C, (0, 2, 1)) - C * np.eye(nb)[None]
-        U = _monomer_rotation(np.asarray(lmn)[:nb], R)
-        D = U @ C @ np.transpose(U, (0, 2, 1))
+        lmn = np.asarray(molecules[ks[0]][2])[:nb]
+        U = _monomer_rotation(lmn, R.reshape(M * nm, 3, 3))
+        D = (U @ C @ np.transpose(U, (0, 2, 1))).reshape(M, nm, nb, nb)
+        P = np.repeat(P0[None], M, axis=0)
         for m in range(nm):
-            P[m * nb:(m + 1) * nb, m * nb:(m + 1) * nb] += D[m]
-        out.append(P)
-        k += nm
+            P[:, m * nb:(m + 1) * nb, m * nb:(m + 1) * nb] += D[:, m]
+        for j, k in enumerate(ks):
+            out[k] = P[j]
     return out
 
 

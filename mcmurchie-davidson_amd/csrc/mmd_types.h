@@ -49,8 +49,12 @@ struct ClassArgs {
 
 // Row decomposition of one quartet class for the Fock kernels.
 // Primitive pairs per slice of the Fock-build class kernels: a work unit
-// takes at most PSLICE bra primitive pairs, a lane at most PSLICE ket ones.
-constexpr int PSLICE = 8;
+// takes at most PSLICE bra primitive pairs (slices of 4 measured slower:
+// more units).
+#ifndef MMD_PSLICE
+#define MMD_PSLICE 8
+#endif
+constexpr int PSLICE = MMD_PSLICE;
 
 // Per work unit: an upper bound of the pair factors q_P q_Q over the unit's
 // quartets (rounded up to float) and the molecule of its bra pair, so that a

@@ -44,19 +44,19 @@ FP64_PEAK_TFLOPS = 78.6
 # counters read for a copy of known size (recorded with the command lines in
 # profiles/); a class absent here reports null.
 TRAFFIC = {
-    (0, 0, 0, 0): 2.239e+07, (1, 0, 0, 0): 4.238e+07, (1, 0, 1, 0): 4.503e+07, (1, 1, 0, 0): 2.112e+07,
-    (1, 1, 1, 0): 5.566e+07, (1, 1, 1, 1): 7.852e+06, (2, 0, 0, 0): 1.826e+07, (2, 0, 1, 0): 3.815e+07,
-    (2, 0, 1, 1): 1.128e+07, (2, 0, 2, 0): 5.933e+06, (2, 1, 0, 0): 1.416e+07, (2, 1, 1, 0): 1.686e+07,
-    (2, 1, 1, 1): 3.417e+07, (2, 1, 2, 0): 3.096e+07, (2, 1, 2, 1): 7.055e+06, (2, 2, 0, 0): 3.465e+06,
-    (2, 2, 1, 0): 4.796e+06, (2, 2, 1, 1): 6.789e+06, (2, 2, 2, 0): 5.410e+06, (2, 2, 2, 1): 3.559e+06,
+    (0, 0, 0, 0): 2.203e+07, (1, 0, 0, 0): 4.100e+07, (1, 0, 1, 0): 4.481e+07, (1, 1, 0, 0): 2.065e+07,
+    (1, 1, 1, 0): 5.334e+07, (1, 1, 1, 1): 7.910e+06, (2, 0, 0, 0): 1.781e+07, (2, 0, 1, 0): 3.797e+07,
+    (2, 0, 1, 1): 1.138e+07, (2, 0, 2, 0): 6.591e+06, (2, 1, 0, 0): 1.420e+07, (2, 1, 1, 0): 1.657e+07,
+    (2, 1, 1, 1): 7.255e+06, (2, 1, 2, 0): 5.512e+06, (2, 1, 2, 1): 5.707e+06, (2, 2, 0, 0): 3.591e+06,
+    (2, 2, 1, 0): 4.837e+06, (2, 2, 1, 1): 2.786e+06, (2, 2, 2, 0): 2.254e+06, (2, 2, 2, 1): 5.863e+06,
     (2, 2, 2, 2): 1.207e+06,
 }
 
 
 # The same for the dominant class kernel of the headline workload.
 TRAFFIC_HEADLINE = {
-    (0, 0, 0, 0): 3.116e+08, (1, 0, 0, 0): 3.929e+08, (1, 0, 1, 0): 1.411e+08, (1, 1, 0, 0): 1.788e+08,
-    (1, 1, 1, 0): 1.162e+08, (1, 1, 1, 1): 1.994e+07,
+    (0, 0, 0, 0): 4.124e+08, (1, 0, 0, 0): 3.689e+08, (1, 0, 1, 0): 1.423e+08, (1, 1, 0, 0): 1.348e+08,
+    (1, 1, 1, 0): 9.868e+07, (1, 1, 1, 1): 1.853e+07,
 }
 
 

@@ -49,10 +49,11 @@ struct ClassArgs {
 
 // Row decomposition of one quartet class for the Fock kernels.
 // Primitive pairs per slice of the Fock-build class kernels: a work unit
-// takes at most PSLICE bra primitive pairs (slices of 4 measured slower:
-// more units).
+// takes at most PSLICE bra primitive pairs (a 64-molecule 6-31G batch pass
+// measured 372, 336, 313 and 313 ms for 4, 8, 16 and 32; (H2O)8 cc-pVDZ
+// builds 1.95, 1.99 and 2.02 ms for 8, 16 and 32).
 #ifndef MMD_PSLICE
-#define MMD_PSLICE 8
+#define MMD_PSLICE 16
 #endif
 constexpr int PSLICE = MMD_PSLICE;
 

@@ -47,6 +47,10 @@ __host__ __device__ constexpr int chunk_size(int nab, int hcd, int ncd, int limi
     return best;
 }
 constexpr int ACC_LIMIT = 40;
+#ifndef MMD_IBUF_CAP
+#define MMD_IBUF_CAP 2048
+#endif
+constexpr int IBUF_CAP = MMD_IBUF_CAP;     // doubles of the LDS integral buffer that set its batch size
 constexpr int ACC_LIMIT_HIGH = 64;
 
 template <int LA, int LB, int LC, int LD>
@@ -81,7 +85,7 @@ struct ClassShape {
     // has several chunks, their ket-side Coulomb blocks are summed in an LDS
     // buffer per batch (KJ) and added to G once per quartet.
     static constexpr bool KJ = !IBUF && NCHUNK > 1;
-    static constexpr int QB = IBUF ? (2048 / NABCD >= 256 ? 256 : (2048 / NABCD >= 128 ? 128 : 64))
+    static constexpr int QB = IBUF ? (IBUF_CAP / NABCD >= 256 ? 256 : (IBUF_CAP / NABCD >= 128 ? 128 : 64))
                                    : (KJ ? (2048 / NCD >= 256 ? 256 : (2048 / NCD >= 128 ? 128 : 64)) : 256);
     static constexpr int IBUF_DOUBLES = IBUF ? QB * NABCD : 0;
     static constexpr int KJ_DOUBLES = KJ ? QB * NCD : 0;   // used when the launch has LDS for it

@@ -80,7 +80,10 @@ struct ClassShape {
     // ket in cc-pVDZ): there a whole-ket lane would carry up to 81 primitive
     // quartets alone and set the length of its batch.  Unbuffered classes
     // digest each slice's partial block directly (the digestion is linear).
-    static constexpr bool KSL = IBUF || (LC <= 1 && LD == 0);
+    // (pp|pp) is sliced too: its units hold few surviving kets in batches of
+    // small molecules, and slices multiply its lane items (a 6-31G batch
+    // pass measured 285 against 298 ms; (H2O)8 cc-pVDZ builds unchanged).
+    static constexpr bool KSL = IBUF || (LC <= 1 && LD == 0) || (LA <= 1 && LC <= 1);
     // The other classes digest each (quartet, chunk) item; when a quartet
     // has several chunks, their ket-side Coulomb blocks are summed in an LDS
     // buffer per batch (KJ) and added to G once per quartet.

@@ -4,8 +4,8 @@
 Default workload (the headline): 512 x (H2O)n with n uniform in 2..6
 (perturbed monomers, random orientations, seed 1), 6-31G, direct RHF to
 ||dP|| < 1e-8.  One step = all 512 molecules from geometry strings to
-converged energies in the full form: batched on the device (groups of 96
-molecules in one block-diagonal basis, three groups in flight, each SCF
+converged energies in the full form: batched on the device (groups of 128
+molecules in one block-diagonal basis, four groups in flight, each SCF
 iteration replayed as a HIP graph), learned initial density, FP64.  Every molecule's energy is checked
 against the batched FP64 core-guess run (|dE| <= 1 uEh), which is checked
 against the plain FP64 path one molecule at a time; the plain path's rate is
@@ -54,8 +54,8 @@ TRAFFIC = {
 
 # The same for the dominant class kernel of the headline workload.
 TRAFFIC_HEADLINE = {
-    (0, 0, 0, 0): 3.325e+08, (1, 0, 0, 0): 3.597e+08, (1, 0, 1, 0): 1.271e+08, (1, 1, 0, 0): 1.348e+08,
-    (1, 1, 1, 0): 9.230e+07, (1, 1, 1, 1): 1.850e+07,
+    (0, 0, 0, 0): 1.001e+09, (1, 0, 0, 0): 8.332e+08, (1, 0, 1, 0): 3.344e+08, (1, 1, 0, 0): 3.333e+08,
+    (1, 1, 1, 0): 2.321e+08, (1, 1, 1, 1): 3.714e+07,
 }
 
 
@@ -728,8 +728,8 @@ def main():
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--serial-classes", action="store_true",
                     help="run the Fock classes one after the other in the profiled step (for counter runs)")
-    ap.add_argument("--group", type=int, default=96, help="molecules per block-diagonal batch (batch workloads)")
-    ap.add_argument("--batch-streams", type=int, default=3, help="groups in flight at once (headline)")
+    ap.add_argument("--group", type=int, default=128, help="molecules per block-diagonal batch (batch workloads)")
+    ap.add_argument("--batch-streams", type=int, default=4, help="groups in flight at once (headline)")
     ap.add_argument("--molecules", type=int, default=512,
                     help="headline molecules (512 is the workload; fewer only for counter runs)")
     ap.add_argument("--class-streams", type=int, default=2,

@@ -490,9 +490,10 @@ def cfg2_extras(args):
 def bench_headline(args):
     """The headline workload: 512 x (H2O)n, n = 2..6, 6-31G.  Full form =
     batched on the device (mmd_batch_rhf, groups of `--group` molecules in one
-    block-diagonal basis, graph-replayed iterations, two groups in flight) +
-    learned start + FP32 tier below 1e-4, from the geometry strings to
-    converged energies.  Checked per molecule against the batched FP64
+    block-diagonal basis, graph-replayed iterations, `--batch-streams` groups
+    in flight, per-class ket slices tuned on one group before the timed
+    steps) + learned start, FP64, from the geometry strings to converged
+    energies.  Checked per molecule against the batched FP64
     core-guess run (|dE| <= 1 uEh), which is checked against the plain FP64
     path one molecule at a time (Molecule(g).RHF(direct=True)) on every 8th
     geometry; the plain path's rate is measured on those."""
@@ -503,11 +504,19 @@ def bench_headline(args):
     G = args.group
     full_kw = dict(group=G, guess="learned", fp32_below=args.fp32_below, streams=args.batch_streams,
                    class_streams=args.class_streams)
+    ctxs = _device_contexts(0, max(1, args.batch_streams))
+    slices = None
+    if not args.no_tune:
+        # per-class ket slices of the row kernels, measured on one group of
+        # the workload's molecules (not timed)
+        from mmd.tune import tune_ket_slices
+        idx = np.linspace(0, len(geoms) - 1, min(G, len(geoms))).astype(int)
+        slices, _ = tune_ket_slices([geoms[i] for i in idx], "6-31g", ctx=ctxs[0], others=ctxs[1:])
+        print("ket slices tuned: %s" % list(slices), file=sys.stderr, flush=True)
     for _ in range(max(1, args.warmup)):
         run_batch_device(geoms[:4 * G], "6-31g", **full_kw)
         run_batch_device(geoms[:2 * G], "6-31g", group=G, streams=args.batch_streams)
     print("warmup done", file=sys.stderr, flush=True)
-    ctxs = _device_contexts(0, max(1, args.batch_streams))
     step_s = []
     full = None
     per = None
@@ -531,7 +540,7 @@ def bench_headline(args):
         out = {"metric": METRIC, "value": None, "unit": "SCF/s", "checked": False,
                "note": "--steps-only: full-form steps without the FP64 and plain checks",
                "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * el_full, "step_s": step_s,
-               "molecules": len(geoms)}
+               "molecules": len(geoms), "ket_slices": None if slices is None else [int(x) for x in slices]}
         if per:
             out["roofline"] = headline_roofline(per)
         return out
@@ -572,6 +581,7 @@ def bench_headline(args):
                    "size_mix": {str(k): ns.count(k) for k in sorted(set(ns))}, "hw_queues": int(HW_QUEUES)},
         "step": "one pass over the %d molecules, geometry strings to converged energies" % len(geoms),
         "step_s": step_s,
+        "ket_slices": None if slices is None else [int(x) for x in slices],
         "max_abs_dE_full_vs_fp64_uEh": float(dE.max() * 1e6),
         "all_converged": bool(all(r["converged"] for r in full)),
         "mean_iterations_full": float(np.mean([r["iterations"] for r in full])),
@@ -726,6 +736,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="headline: keep the default ket slices instead of tuning them on one group")
     ap.add_argument("--serial-classes", action="store_true",
                     help="run the Fock classes one after the other in the profiled step (for counter runs)")
     ap.add_argument("--group", type=int, default=128, help="molecules per block-diagonal batch (batch workloads)")

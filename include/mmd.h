@@ -203,6 +203,14 @@ int mmd_last_gradient_ms(mmd_ctx *ctx, double *ms);
  * per class 0 = row kernel, 1 = matrix cores, -1 = not decided yet.
  */
 int mmd_set_class_impl(mmd_ctx *ctx, int mode, int32_t *choice);
+/*
+ * Ket primitive pairs per lane item of the row kernels' sliced classes, per
+ * class (21 entries in class order, 1..16; 0 keeps a class's value; slices
+ * NULL changes nothing); current (optional, 21 entries) receives the values
+ * in force.  Default 2.  No counterpart in the reference (tuning; see
+ * mmd.tune.tune_ket_slices).
+ */
+int mmd_set_ket_slices(mmd_ctx *ctx, const int32_t *slices, int32_t *current);
 int mmd_set_profiling(mmd_ctx *ctx, int on);
 int mmd_fock_stats(mmd_ctx *ctx, int32_t *lclass, int64_t *nquart, int64_t *nprim, double *ms);
 /*

@@ -235,6 +235,7 @@ struct mmd_ctx {
     // undecided, 0 row, 1 matrix cores).
     int impl_mode = 2;
     int impl_choice[NQCLASS];
+    int ket_slice[NQCLASS];             // mmd_set_ket_slices (KSLICE_DEFAULT until set)
     int tune_phase = 0;
     double tune_ms[2][NQCLASS] = {};
     bool tune_measuring = false;
@@ -898,6 +899,7 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol, FockScratch& W0)
             R.row_counter = W0.rowctr + qc;
             R.maxpp = maxpp[qc];
             R.kj = kj_lds[qc] ? 1 : 0;
+            R.kslice = c->ket_slice[qc];
             R.seg = c->unit_seg[qc];
             R.rw = c->rw;
             R.units = c->d_units.p + c->unit_off[qc];
@@ -1066,7 +1068,10 @@ int mmd_ctx_create(mmd_ctx** out, int device_id)
     mmd_ctx* c = new (std::nothrow) mmd_ctx();
     if (!c) return fail(MMD_ERR_NOMEM, "context allocation failed");
     c->device = device_id;
-    for (int qc = 0; qc < NQCLASS; ++qc) c->impl_choice[qc] = -1;
+    for (int qc = 0; qc < NQCLASS; ++qc) {
+        c->impl_choice[qc] = -1;
+        c->ket_slice[qc] = KSLICE_DEFAULT;
+    }
     // Classes on more streams than the process has hardware queues would
     // share queues in an order the longest-first plan does not see: spread
     // them over min(8, GPU_MAX_HW_QUEUES) streams (HIP's default is 4).  The
@@ -1741,6 +1746,21 @@ int mmd_fock_profile_sums(mmd_ctx* ctx, int32_t* lclass, int64_t* nquart, int64_
         if (builds) builds[qc] = a.builds;
     }
     return NQCLASS;
+}
+
+int mmd_set_ket_slices(mmd_ctx* ctx, const int32_t* slices, int32_t* current)
+{
+    if (!ctx) return fail(MMD_ERR_ARG, "null context");
+    if (slices) {
+        for (int qc = 0; qc < NQCLASS; ++qc)
+            if (slices[qc] != 0 && (slices[qc] < 1 || slices[qc] > KSLICE_MAX))
+                return fail(MMD_ERR_ARG, "class %d: ket slice %d outside 1..%d", qc, (int)slices[qc], KSLICE_MAX);
+        for (int qc = 0; qc < NQCLASS; ++qc)
+            if (slices[qc] != 0) ctx->ket_slice[qc] = slices[qc];
+    }
+    if (current)
+        for (int qc = 0; qc < NQCLASS; ++qc) current[qc] = ctx->ket_slice[qc];
+    return MMD_OK;
 }
 
 int mmd_set_class_impl(mmd_ctx* ctx, int mode, int32_t* choice)

@@ -416,14 +416,7 @@ __device__ __forceinline__ void static_switch(int i, F&& f)
 // whose bra rows do not fit in LDS to the wavefront-per-quartet kernel.)
 // ---------------------------------------------------------------------------
 constexpr int CLASS_BLOCK = 256;           // threads per workgroup
-#ifndef MMD_KSLICE
-#define MMD_KSLICE 2
-#endif
-// Ket primitive pairs per lane item of the sliced classes: short slices
-// keep the lanes of a wavefront even when a batch mixes contraction
-// lengths (a 64-molecule 6-31G batch pass measured 369, 341 and 334 ms for
-// slices of 8, 4 and 2; (H2O)8 cc-pVDZ builds were the same for all three).
-constexpr int KSLICE = MMD_KSLICE;
+
 #ifndef MMD_BOYS_LDS_MIN_L
 #define MMD_BOYS_LDS_MIN_L 0
 #endif
@@ -537,7 +530,7 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
                 // digest the batch qlist[b0, b0+cnt): a wavefront takes one
                 // chunk of bra components and 64 consecutive lane items.
                 // Classes that add partial integrals into the LDS buffer
-                // split each ket's primitives into slices of KSLICE; their
+                // split each ket's primitives into slices of R.kslice; their
                 // lane items are the (quartet, ket slice) pairs that exist,
                 // numbered through the prefix sums pre[] of the quartets'
                 // slice counts, so that lanes stay busy however the batch
@@ -549,7 +542,7 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
                 if constexpr (S::KSL) {
                     const int j = threadIdx.x;   // cnt <= QB <= CLASS_BLOCK
                     int x = 0;
-                    if (j < cnt) x = (A.pairs[R.koff + qlist[b0 + j]].npp + KSLICE - 1) / KSLICE;
+                    if (j < cnt) x = (A.pairs[R.koff + qlist[b0 + j]].npp + R.kslice - 1) / R.kslice;
                     for (int off = 1; off < 64; off <<= 1) {
                         const int y = __shfl_up(x, off);
                         if (lane >= off) x += y;
@@ -582,9 +575,9 @@ __global__ void __launch_bounds__(CLASS_BLOCK) eri_fock_class_kernel(ClassArgs A
                     const int Q = qlist[b0 + idx];
                     const int2 qd = make_int2(R.boff + P, R.koff + Q);
                     const ShellPair ket = A.pairs[qd.y];
-                    const int jq0 = ks * KSLICE;
+                    const int jq0 = ks * R.kslice;
                     if (jq0 >= ket.npp) continue;
-                    const int jq1 = S::KSL ? min(ket.npp, jq0 + KSLICE) : ket.npp;
+                    const int jq1 = S::KSL ? min(ket.npp, jq0 + R.kslice) : ket.npp;
                     const int ip0 = 0, ip1 = bnp;
                     if (chunk == 0) my_nprim += (unsigned long long)(ip1 - ip0) * (unsigned long long)(jq1 - jq0);
                     static_switch<0, S::NCHUNK>(chunk, [&](auto ic) {

@@ -47,6 +47,14 @@ struct ClassArgs {
     double* G;                  // Fock accumulation N x N
 };
 
+// Ket primitive pairs per lane item of the sliced classes, unless set per
+// class (mmd_set_ket_slices): short slices keep the lanes of a wavefront
+// even when a batch mixes contraction lengths (a 64-molecule 6-31G batch
+// pass measured 369, 341 and 334 ms for slices of 8, 4 and 2; (H2O)8
+// cc-pVDZ builds were the same for all three).
+constexpr int KSLICE_DEFAULT = 2;
+constexpr int KSLICE_MAX = 16;
+
 // Row decomposition of one quartet class for the Fock kernels.
 // Primitive pairs per slice of the Fock-build class kernels: a work unit
 // takes at most PSLICE bra primitive pairs (a 64-molecule 6-31G batch pass
@@ -76,6 +84,7 @@ struct RowArgs {
     int nunits;
     int maxpp;                  // largest primitive-pair count of the class's bra pairs
     int kj;                     // LDS holds the ket-side Coulomb buffer (classes that have one)
+    int kslice;                 // ket primitive pairs per lane item of the sliced classes (1 .. 16)
     int nshell;
     const double* Dsh;          // shell-block maxima of |dP| (nshell x nshell)
     const double* mol_dmax;     // batches: max |dP| of each molecule; or null

@@ -86,6 +86,8 @@ lib.mmd_fock_stats.restype = _c_int
 lib.mmd_fock_stats.argtypes = [_ctx_p, _vp, _vp, _vp, _vp]
 lib.mmd_set_class_impl.restype = _c_int
 lib.mmd_set_class_impl.argtypes = [_ctx_p, _c_int, _vp]
+lib.mmd_set_ket_slices.restype = _c_int
+lib.mmd_set_ket_slices.argtypes = [_ctx_p, _vp, _vp]
 lib.mmd_rhf_gradient.restype = _c_int
 lib.mmd_rhf_gradient.argtypes = [_ctx_p, _vp, _vp, _vp, _vp]
 lib.mmd_last_gradient_ms.restype = _c_int
@@ -133,7 +135,7 @@ EXPORTED = [
     "mmd_last_error", "mmd_version", "mmd_device_count", "mmd_ctx_create", "mmd_ctx_destroy",
     "mmd_set_basis", "mmd_set_atoms", "mmd_one_electron", "mmd_schwarz", "mmd_set_schwarz",
     "mmd_eri_tensor", "mmd_eri_list", "mmd_fock_delta", "mmd_fock_delta_noalloc", "mmd_jk_tensor", "mmd_boys",
-    "mmd_last_fock_ms", "mmd_set_profiling", "mmd_fock_stats", "mmd_fock_profile_sums", "mmd_rhf_gradient", "mmd_set_class_impl",
+    "mmd_last_fock_ms", "mmd_set_profiling", "mmd_fock_stats", "mmd_fock_profile_sums", "mmd_rhf_gradient", "mmd_set_class_impl", "mmd_set_ket_slices",
     "mmd_last_gradient_ms", "mmd_set_precision", "mmd_fock_tiers",
     "mmd_set_streams", "mmd_set_lds_limit", "mmd_overlap_powers", "mmd_dgemm", "mmd_set_molecules",
     "mmd_batch_rhf", "mmd_scf_begin", "mmd_scf_step", "mmd_scf_density", "mmd_scf_get", "mmd_ao2mo",
@@ -367,6 +369,17 @@ class Context:
         ch = np.zeros(21, dtype=np.int32)
         check(lib.mmd_set_class_impl(self._h, -1 if mode is None else int(mode), _ptr(ch)))
         return ch
+
+    def set_ket_slices(self, slices=None):
+        """Ket primitive pairs per lane item of the row kernels' sliced
+        classes: slices (21 entries, 0 keeps a class's value) or None;
+        returns the values in force."""
+        cur = np.zeros(21, dtype=np.int32)
+        src = None if slices is None else np.ascontiguousarray(slices, dtype=np.int32)
+        if src is not None and src.size != 21:
+            raise ValueError("21 entries expected")
+        check(lib.mmd_set_ket_slices(self._h, None if src is None else _ptr(src), _ptr(cur)))
+        return cur
 
     def rhf_gradient(self, P, W, bf_atom):
         """Electronic part of the RHF nuclear gradient (natom, 3) for the

@@ -108,3 +108,31 @@ def test_batch_input_edges(native):
     assert run_batch_device([], "6-31g") == []
     with pytest.raises(Exception, match="basis functions"):
         run_batch_device([water_cluster(4, 7.0, seed=3)], "cc-pvdz", group=1)
+
+
+def test_ket_slices_keep_energies_and_tuner_sets_valid_slices(native):
+    """The ket slice of the row kernels only regroups the primitive sums:
+    batches with slices of 1 and of 8 give the same energies (1e-10 Eh) and
+    iterations as the default; the tuner sets a valid slice on every class
+    and the batch with its choice gives the same energies again."""
+    from mmd import _native
+    from mmd.batch import run_batch_device
+    from mmd.tune import tune_ket_slices
+    geoms = [water_cluster(k, 2.0 + 1.4 * k, seed=300 + k, perturb=True) for k in (2, 3, 4, 5, 2, 6)]
+    ctx = _native.Context(0)
+    base = run_batch_device(geoms, "6-31g", guess="sad", ctx=ctx)
+    for k in (1, 8):
+        ctx.set_ket_slices(np.full(21, k, dtype=np.int32))
+        assert list(ctx.set_ket_slices()) == [k] * 21
+        got = run_batch_device(geoms, "6-31g", guess="sad", ctx=ctx)
+        for a, b in zip(base, got):
+            assert a["iterations"] == b["iterations"]
+            assert abs(a["energy"] - b["energy"]) < 1e-10
+    with pytest.raises(ValueError):
+        ctx.set_ket_slices(np.full(21, 17, dtype=np.int32))
+    slices, per = tune_ket_slices(geoms, "6-31g", ctx=ctx, guess="sad")
+    assert list(ctx.set_ket_slices()) == list(slices)
+    assert all(s in (1, 2, 4, 8) for s in slices) and set(per) == {1, 2, 4, 8}
+    got = run_batch_device(geoms, "6-31g", guess="sad", ctx=ctx)
+    for a, b in zip(base, got):
+        assert abs(a["energy"] - b["energy"]) < 1e-10

@@ -54,8 +54,8 @@ TRAFFIC = {
 
 # The same for the dominant class kernel of the headline workload.
 TRAFFIC_HEADLINE = {
-    (0, 0, 0, 0): 1.001e+09, (1, 0, 0, 0): 8.332e+08, (1, 0, 1, 0): 3.344e+08, (1, 1, 0, 0): 3.333e+08,
-    (1, 1, 1, 0): 2.321e+08, (1, 1, 1, 1): 3.714e+07,
+    (0, 0, 0, 0): 1.001e+09, (1, 0, 0, 0): 8.355e+08, (1, 0, 1, 0): 3.344e+08, (1, 1, 0, 0): 3.327e+08,
+    (1, 1, 1, 0): 2.322e+08, (1, 1, 1, 1): 3.710e+07,
 }
 
 

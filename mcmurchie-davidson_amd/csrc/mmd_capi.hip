@@ -927,6 +927,11 @@ int run_fock_classes(mmd_ctx* c, ClassArgs A, double tol, FockScratch& W0)
                 R.row_counter = W0.rowctr + NQCLASS + qc;
                 R.stat_nquart = W0.nprim + 2 * NQCLASS + qc;
                 R.stat_nprim = c->profiling ? W0.nprim + 3 * NQCLASS + qc : nullptr;
+                // the second traversal holds the many small-bound quartets
+                // of a unit: longer ket slices keep its items few ((H2O)16
+                // cc-pVDZ tier build 9.3 ms with the FP64 pass's slices of 2,
+                // 8.8 ms with 8)
+                R.kslice = std::max(R.kslice, 8);
                 HIP_TRY(launch_fock_class(qc, 1, A, R, grid, sq));
             }
         } else {
